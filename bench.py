@@ -1,0 +1,162 @@
+"""Throughput of the B1 fan-beam projector pair on one MI355X.
+
+One step = one A (image -> sinogram) plus one A^T (sinogram -> image) of the
+bilinear (B1) box-spline projector at 512 x 512 pixels (0.6 mm), 720 views
+over 2 pi, 768 bins (0.8 mm), source-isocentre 500 mm, source-detector
+1000 mm, FP32, inputs resident in HBM.  The image is a seeded random-ellipse
+phantom; A^T is applied to its Poisson-noisy (1e5 photons) log sinogram.
+
+Prints one JSON line (rank 0).  `roofline` prices the dominant kernel with the
+view-visit byte model 4 np (nx ny + nu) bytes per launch (every view reads or
+updates every pixel once and its detector row once), timed per launch with
+HIP events on the launch stream.  `cpu_baseline` times the CPU restatement of
+the reference formulas (FP32 footprint, FP64 sums, OpenMP) on a bounded
+sample of views and scales to pairs/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NX = NY = 512
+NP_VIEWS = 720
+NU = 768
+DX = 0.6
+DU = 0.8
+LSD = 1000.0
+LSO = 500.0
+HBM_PEAK_GBS = 8000.0
+METRIC = "A+Aᵀ pairs/s at 512²/720v (%HBM peak)"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--basis", default="B1", choices=["B1", "B0", "SF"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-views", type=int, default=180)
+    return ap.parse_args()
+
+
+def cpu_baseline(img, sino, ang, basis, nviews):
+    """Oracle in reference mode on views [0, nviews): A and A^T, scaled."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Geometry, Oracle
+
+    g = Geometry(Oracle("ref"), NX, NY, ang, NU, DX, DX, DU, LSD, LSO)
+    nviews = min(nviews, NP_VIEWS)
+    t0 = time.perf_counter()
+    g.forward(basis, img, views=(0, nviews))
+    g.backward(basis, sino, views=(0, nviews))
+    dt = time.perf_counter() - t0
+    per_pair = dt * NP_VIEWS / nviews
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": 1.0 / per_pair,
+        "unit": "pairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%s A + A^T over %d of %d views at 512x512/768 bins (%.2f s), scaled to one full pair; "
+                  "oracle/sgdct_oracle.c FP32-footprint build, OpenMP %d threads"
+                  % (basis, nviews, NP_VIEWS, dt, threads),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(views would shard across GPUs with one all-reduce of A^T; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from sgd_amd.phantom import phantom, poisson_noise
+    from sgd_amd.torch_ops import Projector
+
+    dev = torch.device("cuda", 0)
+    ang = 2 * np.pi * np.arange(NP_VIEWS) / NP_VIEWS
+    P = Projector(args.basis, NX, NY, ang, NU, DX, DX, DU, LSD, LSO)
+    img_np = phantom(0, NX, NY, DX, DX)
+    x = torch.tensor(img_np, dtype=torch.float32, device=dev)
+    y0 = P.forward(x)
+    sino_np = poisson_noise(y0.double().cpu().numpy() * 0.1, 1e5, seed=1) * 10.0
+    y = torch.tensor(sino_np, dtype=torch.float32, device=dev)
+    yo = torch.empty_like(y)
+    xo = torch.empty_like(x)
+    stream = torch.cuda.current_stream(dev)
+
+    for _ in range(args.warmup):
+        P.forward(x, out=yo)
+        P.backward(y, out=xo)
+    torch.cuda.synchronize()
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        e = ev[k]
+        e[0].record(stream)
+        P.forward(x, out=yo)
+        e[1].record(stream)
+        e[2].record(stream)
+        P.backward(y, out=xo)
+        e[3].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    fwd_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
+    bwd_ms = sum(e[2].elapsed_time(e[3]) for e in ev) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    value = args.steps / wall
+    bytes_per_launch = 4.0 * NP_VIEWS * (NX * NY + NU)
+    dom_ms, dom = (fwd_ms, "A") if fwd_ms >= bwd_ms else (bwd_ms, "A^T")
+    achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
+    evals = P.eval_count()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%s fan-beam A + A^T pair, 512x512 image (0.6 mm), 720 views over 2pi, 768 bins "
+                        "(0.8 mm), lso 500 mm, lsd 1000 mm" % args.basis,
+            "basis": args.basis, "nx": NX, "ny": NY, "views": NP_VIEWS, "bins": NU,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": dom,
+            "bytes_per_launch": bytes_per_launch,
+        },
+        "kernel_ms": {"A": fwd_ms, "A^T": bwd_ms},
+        "footprint_evals_per_A": evals,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(img_np, sino_np, ang, args.basis, args.cpu_views)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

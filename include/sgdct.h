@@ -1,0 +1,84 @@
+/*
+ * sgdct.h -- C ABI of the MI355X (gfx950) box-spline footprint projectors.
+ *
+ * Drop-in for the extern "C" surface of hqpwca/SGD's projector layers.  The
+ * reference-named entry points keep the reference's argument order, meaning
+ * and host layouts:
+ *   image     x[ix*ny + iy]  (FP64, host)       layers/A_B1.cu:621-623
+ *   sinogram  b[p*nu + u]    (FP64, host)       layers/A_B1.cu:639-641
+ *   A overwrites b, A^T overwrites x.
+ * Differences from the reference, all additive:
+ *   - dz / drho are explicit nullable pointers (the reference relies on C++
+ *     default arguments that do not exist across a C ABI, A_B1.hh:24).
+ *   - ny is honoured (the reference reads ny = nx, layers/A_B1.cu:607).
+ *   - errors are negative return codes, never exceptions; sgdct_last_error()
+ *     describes the last one on the calling thread.
+ *   - a destroy function, and FP32 device-pointer entry points on a caller
+ *     stream for callers that keep data on the GPU.
+ */
+#ifndef SGDCT_H
+#define SGDCT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sgdct_projector sgdct_projector;
+
+/* Error codes (negative). */
+#define SGDCT_OK 0
+#define SGDCT_EINVAL (-1)   /* bad argument or shape */
+#define SGDCT_EDEVICE (-2)  /* HIP runtime error (no device, launch failure) */
+#define SGDCT_ENOMEM (-3)   /* device allocation failed */
+#define SGDCT_EUNSUP (-4)   /* basis / operation not supported */
+
+/* ---- reference-named entry points -------------------------------------- */
+
+/* layers/A_B0.hh:24 (b0_init), A_B0.cu:504 (forward), A_B0.cu:543 (backward) */
+sgdct_projector *b0_init(int nx, int ny, int np, int nu, double dx, double dy, double du,
+                         double lsd, double lso, const double *angles, const double *dz,
+                         const double *drho);
+int b0_forward_projection(double *b, double *x, sgdct_projector *h);
+int b0_backward_projection(double *b, double *x, sgdct_projector *h);
+
+/* layers/A_B1.hh:24 (b1_init), A_B1.cu:605 (forward), A_B1.cu:646 (backward) */
+sgdct_projector *b1_init(int nx, int ny, int np, int nu, double dx, double dy, double du,
+                         double lsd, double lso, const double *angles, const double *dz,
+                         const double *drho);
+int b1_forward_projection(double *b, double *x, sgdct_projector *h);
+int b1_backward_projection(double *b, double *x, sgdct_projector *h);
+
+/* layers/A_SF.hh (SF_init), A_SF.cu:396 (init), :406 (forward), :447 (backward) */
+sgdct_projector *SF_init(int nx, int ny, int np, int nu, double dx, double dy, double du,
+                         double lsd, double lso, const double *angles);
+int SF_forward_projection(double *b, double *x, sgdct_projector *h);
+int SF_backward_projection(double *b, double *x, sgdct_projector *h);
+
+/* ---- additions ------------------------------------------------------------ */
+
+/* Release a handle from any *_init (the reference leaks them). */
+void sgdct_destroy(sgdct_projector *h);
+
+/* Image (ny, nx) FP32 row-major [iy][ix] and sinogram (np, nu) FP32 [p][u],
+ * both device pointers; `batch` images / sinograms back to back; `stream` is
+ * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.
+ * Returns SGDCT_OK or a negative code. */
+int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
+int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
+
+/* Shape query: out[0..3] = nx, ny, np, nu; returns the basis id
+ * (0 B0, 1 B1, 2 SF) or a negative code. */
+int sgdct_shape(const sgdct_projector *h, int *out);
+
+/* Footprint evaluations of one A under the reference's corner rule (the
+ * count layers/A_B1.cu:643 returns), as 64-bit; negative on error. */
+long long sgdct_eval_count(sgdct_projector *h);
+
+/* Human-readable description of the calling thread's last error. */
+const char *sgdct_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGDCT_H */

@@ -1,0 +1,786 @@
+/*
+ * sgdct_oracle.c -- CPU restatement of the reference's fan-beam footprint
+ * projectors (B0, B1, SF, H0, Hl) and an FP64 strip-integral oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker or
+ * as the timed CPU baseline.  The product (sgd_amd/) never links or calls it.
+ *
+ * The file is compiled twice (see oracle/Makefile):
+ *   -DORACLE_FP=float   "reference mode": the footprint arithmetic is FP32
+ *                       exactly where layers/A_*.cu compute in float, with the
+ *                       same expression forms and evaluation order, and FP64
+ *                       accumulation where the reference accumulates in double.
+ *   -DORACLE_FP=double  "exact mode": the same formulas evaluated in FP64.
+ * The closed-form polynomials below are kept term-for-term equal to the
+ * reference's (layers/spline_conv.hh) on purpose: reference mode is meant to
+ * reproduce the reference's FP32 rounding, which depends on expression form.
+ *
+ * Image layout: img[iy*nx + ix]; sinogram layout: sino[p*nu + u].
+ *
+ * Pins (see tests/test_oracle_*.py): closed-form <-> truncated-power
+ * identities, unit integrals and symmetry of every 1-D footprint, brute-force
+ * exact strip integrals (or_strip_exact) on small inputs, adjointness,
+ * 90-degree rotation <-> view shift, per-view mass.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef ORACLE_FP
+#define ORACLE_FP float
+#endif
+typedef ORACLE_FP fpt;
+
+#define OR_PI 3.14159265358979323846
+#define EPS7 ((fpt)1e-7)
+
+static inline fpt fp_abs(fpt v) { return v < 0 ? -v : v; }
+static inline fpt fp_max(fpt a, fpt b) { return a > b ? a : b; }
+static inline fpt fp_min(fpt a, fpt b) { return a < b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline int fp_floor_int(fpt v) { return (int)floor((double)v); }
+
+/* ------------------------------------------------------------------------- */
+/* Geometry: restates utils/geo.cu:232-269 (source, detector centre and      */
+/* detector step per view, 45-degree snapping) and the projective map of     */
+/* utils/geo.cu:58-161 in closed form.                                       */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny, np, nu;
+    double dx, dy, du, lsd, lso;
+    double *src; /* np*2 */
+    double *dtv; /* np*2 */
+    double *puv; /* np*2 */
+    double *pm;  /* np*12, projective map (x, y, z, 1) -> u, rows 0..2 */
+    fpt *pmi;    /* np*12, pm pre-multiplied for integer pixel indices */
+} ogeo;
+
+/* Per-view source/detector placement (utils/geo.cu:240-262). */
+static void view_placement(double angle, double lsd, double lso, double du, double drho,
+                           double *src, double *dtv, double *puv) {
+    double beta = angle;
+    double s = sin(beta), c = cos(beta);
+    /* snap views within 1e-7 of the 45-degree diagonals (utils/geo.cu:254-257) */
+    if (fabs(s - c) < 1e-7)
+        beta = (s > 0) ? OR_PI / 4 : -OR_PI * 3 / 4;
+    else if (fabs(s + c) < 1e-7)
+        beta = (s > 0) ? OR_PI * 3 / 4 : -OR_PI / 4;
+    s = sin(beta);
+    c = cos(beta);
+    double sx = -lso + drho;
+    src[0] = sx * c;
+    src[1] = sx * s;
+    dtv[0] = (lsd - lso) * c;
+    dtv[1] = (lsd - lso) * s;
+    puv[0] = -du * s;
+    puv[1] = du * c;
+}
+
+/* Closed-form projective map u(X) = nu/2 + (L/du) ((X-src).t)/((X-src).e)
+ * with e the unit source->detector axis, t = puv/du, L the source-detector
+ * distance; rows scaled so that pm[11] == 1 (utils/geo.cu:135).  The
+ * reference builds the same map by matrix products and a 3x3 inverse, with a
+ * special identity fallback on the 3pi/4, 7pi/4 diagonals that evaluates to
+ * the same u; the closed form needs no fallback. */
+static void view_pm(const double *src, const double *dtv, const double *puv, double du, int nu,
+                    int nx, int ny, double dx, double dy, double *pm, fpt *pmi) {
+    double ex = dtv[0] - src[0], ey = dtv[1] - src[1];
+    double L = sqrt(ex * ex + ey * ey);
+    ex /= L;
+    ey /= L;
+    double tx = puv[0] / du, ty = puv[1] / du;
+    double uc = nu / 2.0, k = L / du;
+    double d0 = -(src[0] * ex + src[1] * ey);   /* constant term of row 2 */
+    double n0 = -(src[0] * tx + src[1] * ty);
+    double row0[4] = {uc * ex + k * tx, uc * ey + k * ty, 0.0, uc * d0 + k * n0};
+    double row1[4] = {0.5 * ex, 0.5 * ey, k, 0.5 * d0};
+    double row2[4] = {ex, ey, 0.0, d0};
+    for (int j = 0; j < 4; ++j) {
+        pm[j] = row0[j] / d0;
+        pm[4 + j] = row1[j] / d0;
+        pm[8 + j] = row2[j] / d0;
+    }
+    /* index-space version (utils/geo.cu:151-160; ox, oy in float there) */
+    fpt fox = (fpt)((nx - 1) * (float)dx / 2.0f);
+    fpt foy = (fpt)((ny - 1) * (float)dy / 2.0f);
+    for (int r = 0; r < 3; ++r) {
+        pmi[4 * r + 0] = (fpt)(dx * pm[4 * r + 0]);
+        pmi[4 * r + 1] = (fpt)(dy * pm[4 * r + 1]);
+        pmi[4 * r + 2] = (fpt)(1.0 * pm[4 * r + 2]);
+        pmi[4 * r + 3] = (fpt)(pm[4 * r + 3] - (double)fox * pm[4 * r + 0] - (double)foy * pm[4 * r + 1]);
+    }
+}
+
+void *or_geo_create(int nx, int ny, int np, int nu, double dx, double dy, double du, double lsd,
+                    double lso, const double *angles, const double *drho) {
+    ogeo *g = (ogeo *)calloc(1, sizeof(ogeo));
+    g->nx = nx; g->ny = ny; g->np = np; g->nu = nu;
+    g->dx = dx; g->dy = dy; g->du = du; g->lsd = lsd; g->lso = lso;
+    g->src = (double *)malloc(sizeof(double) * 2 * np);
+    g->dtv = (double *)malloc(sizeof(double) * 2 * np);
+    g->puv = (double *)malloc(sizeof(double) * 2 * np);
+    g->pm = (double *)malloc(sizeof(double) * 12 * np);
+    g->pmi = (fpt *)malloc(sizeof(fpt) * 12 * np);
+    for (int p = 0; p < np; ++p) {
+        view_placement(angles[p], lsd, lso, du, drho ? drho[p] : 0.0, g->src + 2 * p,
+                       g->dtv + 2 * p, g->puv + 2 * p);
+        view_pm(g->src + 2 * p, g->dtv + 2 * p, g->puv + 2 * p, du, nu, nx, ny, dx, dy,
+                g->pm + 12 * p, g->pmi + 12 * p);
+    }
+    return g;
+}
+
+void or_geo_destroy(void *h) {
+    ogeo *g = (ogeo *)h;
+    if (!g) return;
+    free(g->src); free(g->dtv); free(g->puv); free(g->pm); free(g->pmi);
+    free(g);
+}
+
+/* Exported for the geometry tests: continuous detector coordinate of (x, y). */
+double or_geo_u(const void *h, int p, double x, double y) {
+    const ogeo *g = (const ogeo *)h;
+    const double *pm = g->pm + 12 * p;
+    return (pm[0] * x + pm[1] * y + pm[3]) / (pm[8] * x + pm[9] * y + pm[11]);
+}
+
+void or_geo_view(const void *h, int p, double *src, double *dtv, double *puv) {
+    const ogeo *g = (const ogeo *)h;
+    src[0] = g->src[2 * p]; src[1] = g->src[2 * p + 1];
+    dtv[0] = g->dtv[2 * p]; dtv[1] = g->dtv[2 * p + 1];
+    puv[0] = g->puv[2 * p]; puv[1] = g->puv[2 * p + 1];
+}
+
+/* Bin range of a pixel from the projections of four corners at index offset
+ * +-h (h = 1 for B1, 1/2 for B0 and SF), sorted, floored and clamped
+ * (layers/A_B1.cu:100-151, layers/A_B0.cu:158-195).  us[] gets the sorted
+ * corner projections (SF uses them as its trapezoid). */
+static void corner_range(const fpt *pm, int ix, int iy, fpt h, int nu, fpt *us, int *minu,
+                         int *maxu) {
+    fpt sx1 = (fpt)ix - h, sx2 = (fpt)ix + h;
+    fpt sy1 = (fpt)iy - h, sy2 = (fpt)iy + h;
+    us[0] = (pm[0] * sx1 + pm[3] + pm[1] * sy1) / (pm[8] * sx1 + pm[11] + pm[9] * sy1);
+    us[1] = (pm[0] * sx1 + pm[3] + pm[1] * sy2) / (pm[8] * sx1 + pm[11] + pm[9] * sy2);
+    us[2] = (pm[0] * sx2 + pm[3] + pm[1] * sy1) / (pm[8] * sx2 + pm[11] + pm[9] * sy1);
+    us[3] = (pm[0] * sx2 + pm[3] + pm[1] * sy2) / (pm[8] * sx2 + pm[11] + pm[9] * sy2);
+    /* sorting network of layers/spline_conv.hh:24-39 */
+#define OR_SORT2(a, b) do { if ((a) > (b)) { fpt t_ = (b); (b) = (a); (a) = t_; } } while (0)
+    OR_SORT2(us[0], us[1]);
+    OR_SORT2(us[2], us[3]);
+    OR_SORT2(us[0], us[2]);
+    OR_SORT2(us[1], us[3]);
+    OR_SORT2(us[1], us[2]);
+#undef OR_SORT2
+    *minu = imin(imax(0, fp_floor_int(us[0])), nu - 1);
+    *maxu = imin(imax(0, fp_floor_int(us[3])), nu - 1);
+}
+
+/* ------------------------------------------------------------------------- */
+/* 1-D footprints (layers/spline_conv.hh)                                     */
+/* ------------------------------------------------------------------------- */
+
+/* univariate3dirboxspline_fast2, spline_conv.hh:239-242 (xi sorted desc.) */
+static fpt box3_fast2(fpt x, const fpt *xi) {
+    return fp_max((fpt)0.0f, fp_min((fpt)1.0f / xi[0], ((xi[0] + xi[1]) / (fpt)2.0f - x) / (xi[0] * xi[1])));
+}
+
+/* univariate3dirboxspline_fast3, spline_conv.hh:244-293: three unit-integral
+ * boxes of widths xi[0] >= xi[1] >= xi[2] convolved, x = |offset from centre| */
+static fpt box3_fast3(fpt x, const fpt *xi) {
+    fpt h = (fpt)1.0f / xi[0];
+    fpt p = xi[0] * xi[1] * xi[2];
+    fpt K0 = (xi[0] - xi[1] - xi[2]) / (fpt)2.0f;
+    fpt K1 = (xi[0] - xi[1] + xi[2]) / (fpt)2.0f;
+    fpt K2 = (xi[0] + xi[1] - xi[2]) / (fpt)2.0f;
+    fpt K3 = (xi[0] + xi[1] + xi[2]) / (fpt)2.0f;
+    if (x >= K3) return (fpt)0.0f;
+    if (x >= K2) return (fpt)0.5f * (K3 - x) * (K3 - x) / p;
+    if (x >= K1) return ((xi[0] + xi[1]) / (fpt)2.0f - x) * xi[2] / p;
+    if (x < -K0) return h - (K0 * K0 + x * x) / p;
+    if (x < K0) return h;
+    return h - (fpt)0.5f * (K0 - x) * (K0 - x) / p;
+}
+
+/* B0 footprint: layers/A_B0.cu:20-35 (linear_convolution_1d). */
+static fpt b0_conv(fpt y, fpt v1, fpt v2, fpt v3) {
+    fpt vs[3];
+    y = fp_abs(y - (v1 + v2 + v3) / (fpt)2.0f);
+    vs[0] = fp_abs(v1); vs[1] = fp_abs(v2); vs[2] = fp_abs(v3);
+    /* sort3d, descending (spline_conv.hh:52-58) */
+#define OR_SORT2D(a, b) do { if ((a) < (b)) { fpt t_ = (b); (b) = (a); (a) = t_; } } while (0)
+    OR_SORT2D(vs[1], vs[2]);
+    OR_SORT2D(vs[0], vs[1]);
+    OR_SORT2D(vs[1], vs[2]);
+#undef OR_SORT2D
+    if (vs[2] > EPS7) return box3_fast3(y, vs);
+    return box3_fast2(y, vs);
+}
+
+/* Antiderivative from 0 of tri_a * tri_b (unit-integral hats, half-widths
+ * a >= b), branch a > 2b: spline_conv.hh:295-332. */
+static fpt s2tri_ag2b(fpt a, fpt b, fpt x) {
+    fpt a3 = a * a * a, ab2 = a * b * b, b3 = b * b * b;
+    fpt k1 = (6 * a * a * b * b);
+    fpt sum = (fpt)0.0f;
+    if (x <= b) return (x * (12 * ab2 - 4 * b3 - 4 * b * x * x + x * x * x)) / (2 * k1);
+    sum += ((12 * a - 7 * b) * b) / (12 * a * a);
+    if (x <= a - b) return sum + (-2 * a * b + b * b + 2 * a * x - x * x) / (2 * a * a);
+    sum += (fpt)0.5f - b / a;
+    if (x <= a)
+        return sum + ((a - b - x) * (a3 - 13 * b3 + 9 * b * b * x - 3 * b * x * x - x * x * x -
+                                     3 * a * a * (b + x) - 3 * a * (b - x) * (3 * b + x))) / (4 * k1);
+    sum += (13 * b * b) / (24 * a * a);
+    if (x < a + b)
+        return sum + ((x - a) * (a + 2 * b - x) * (a * a + 2 * a * b + 2 * b * b - 2 * (a + b) * x + x * x)) / (4 * k1);
+    return sum + b * b / (24 * a * a);
+}
+
+/* Branch a <= 2b: spline_conv.hh:334-370. */
+static fpt s2tri_al2b(fpt a, fpt b, fpt x) {
+    fpt a3 = a * a * a, a2b = a * a * b, ab2 = a * b * b, b3 = b * b * b;
+    fpt k1 = (6 * a * a * b * b);
+    fpt k2 = a - b;
+    fpt sum = (fpt)0.0f;
+    if (x <= a - b) return (x * (12 * ab2 - 4 * b3 - 4 * b * x * x + x * x * x)) / (2 * k1);
+    sum += ((a - b) * (a3 - 7 * a2b + 23 * ab2 - 9 * b3)) / (2 * k1);
+    if (x <= b)
+        return sum + (((x - a + b) * (a3 - 11 * a2b + 43 * ab2 - 17 * b3 + (5 * a - 13 * b) * k2 * x -
+                                      (a + 7 * b) * x * x + 3 * x * x * x)) / (4 * k1));
+    sum += (((2 * b - a) * (a3 - 6 * a2b + 24 * ab2 - 8 * b3)) / (4 * k1));
+    if (x <= a)
+        return sum + (((b - x) * (4 * a3 - 3 * b3 + b * b * x - 5 * b * x * x - x * x * x -
+                                  6 * a * a * (3 * b + x) + 4 * a * (b * b + 4 * b * x + x * x))) / (4 * k1));
+    sum += (((b - a) * (a3 - 7 * a2b + 5 * ab2 - 3 * b3)) / (4 * k1));
+    if (x < a + b)
+        return sum + ((x - a) * (a + 2 * b - x) * (a * a + 2 * a * b + 2 * b * b - 2 * (a + b) * x + x * x)) / (4 * k1);
+    return sum + b * b / (24 * a * a);
+}
+
+/* fast_2tri_1box_spline, spline_conv.hh:372-390 */
+static fpt tri2box(fpt a, fpt b, fpt x1, fpt x2) {
+    fpt s1, s2;
+    int side = ((x1 * x2) > 0);
+    x1 = fp_abs(x1); x2 = fp_abs(x2);
+    if (b <= a - b) { s1 = s2tri_ag2b(a, b, x1); s2 = s2tri_ag2b(a, b, x2); }
+    else { s1 = s2tri_al2b(a, b, x1); s2 = s2tri_al2b(a, b, x2); }
+    return side ? fp_abs(s1 - s2) : (s1 + s2);
+}
+
+/* fast_1tri_1box_spline, spline_conv.hh:393-405 */
+static fpt tri1box(fpt a, fpt x1, fpt x2) {
+    int side = ((x1 * x2) > 0);
+    x1 = fp_abs(x1); x2 = fp_abs(x2);
+    fpt s1 = (x1 >= a) ? (fpt)0.5f : (x1 / a - (x1 * x1) / (2 * a * a));
+    fpt s2 = (x2 >= a) ? (fpt)0.5f : (x2 / a - (x2 * x2) / (2 * a * a));
+    return side ? fp_abs(s1 - s2) : (s1 + s2);
+}
+
+/* B1 footprint: layers/A_B1.cu:48-59 (linear_convolution_1d). */
+static fpt b1_conv(fpt x1, fpt x2, fpt a, fpt b) {
+    if (b > EPS7) return tri2box(a, b, x1, x2) / fp_abs(x2 - x1);
+    return tri1box(a, x1, x2) / fp_abs(x2 - x1);
+}
+
+/* exported primitives for the footprint tests */
+double or_b0_conv(double y, double v1, double v2, double v3) {
+    return (double)b0_conv((fpt)y, (fpt)v1, (fpt)v2, (fpt)v3);
+}
+double or_b1_conv(double x1, double x2, double a, double b) {
+    return (double)b1_conv((fpt)x1, (fpt)x2, (fpt)a, (fpt)b);
+}
+double or_s2tri(double a, double b, double x) {
+    return (double)((b <= a - b) ? s2tri_ag2b((fpt)a, (fpt)b, (fpt)x) : s2tri_al2b((fpt)a, (fpt)b, (fpt)x));
+}
+double or_box3(double y, double v1, double v2, double v3) {
+    /* centred three-direction box spline value at |y| (no re-centring) */
+    fpt vs[3] = {(fpt)fabs(v1), (fpt)fabs(v2), (fpt)fabs(v3)};
+    fpt t;
+    if (vs[0] < vs[1]) { t = vs[0]; vs[0] = vs[1]; vs[1] = t; }
+    if (vs[1] < vs[2]) { t = vs[1]; vs[1] = vs[2]; vs[2] = t; }
+    if (vs[0] < vs[1]) { t = vs[0]; vs[0] = vs[1]; vs[1] = t; }
+    if (vs[2] > EPS7) return box3_fast3((fpt)fabs(y), vs);
+    return box3_fast2((fpt)fabs(y), vs);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Per-(view, bin) tables                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* B1 vx, vy: layers/A_B1.cu:429-478 (FP64, stored as fpt). */
+static void b1_vectors(const ogeo *g, int p, fpt *vx, fpt *vy) {
+    double cx = g->dtv[2 * p], cy = g->dtv[2 * p + 1];
+    double ux = g->puv[2 * p], uy = g->puv[2 * p + 1];
+    double px = g->src[2 * p], py = g->src[2 * p + 1];
+    for (int u = 0; u < g->nu; ++u) {
+        double sx = cx + ux * (u - 0.5 * g->nu + 0.5);
+        double sy = cy + uy * (u - 0.5 * g->nu + 0.5);
+        double th = atan2(fabs(px - sx), fabs(py - sy));
+        vx[u] = (fpt)fabs(g->dx * cos(th));
+        vy[u] = (fpt)fabs(g->dy * sin(th));
+    }
+}
+
+/* B0 signed vx, vy and bin-edge offsets: layers/A_B0.cu:338-388 (FP64). */
+static void b0_vectors(const ogeo *g, int p, double *tab /* 5*nu */) {
+    int nu = g->nu;
+    double cx = g->dtv[2 * p], cy = g->dtv[2 * p + 1];
+    double ux = g->puv[2 * p], uy = g->puv[2 * p + 1];
+    double px = g->src[2 * p], py = g->src[2 * p + 1];
+    for (int u = 0; u < nu; ++u) {
+        double sx = cx + ux * (u - 0.5 * nu + 0.5);
+        double sy = cy + uy * (u - 0.5 * nu + 0.5);
+        double ax = sx - ux * 0.5, ay = sy - uy * 0.5;
+        double bx = sx + ux * 0.5, by = sy + uy * 0.5;
+        double lsp = sqrt((px - sx) * (px - sx) + (py - sy) * (py - sy));
+        double th = atan2(fabs(px - sx), fabs(py - sy));
+        double vx = fabs(g->dx * cos(th)), vy = fabs(g->dy * sin(th));
+        if (py < sy) vx = -vx;
+        if (px < sx) vy = -vy;
+        double a0s = fabs((sx - px) * (ay - py) - (ax - px) * (sy - py)) / lsp;
+        double sb0 = fabs((sx - px) * (by - py) - (bx - px) * (sy - py)) / lsp;
+        a0s /= ((ax - px) * (sx - px) + (ay - py) * (sy - py)) / (lsp * lsp);
+        sb0 /= ((bx - px) * (sx - px) + (by - py) * (sy - py)) / (lsp * lsp);
+        tab[u] = vx; tab[nu + u] = vy; tab[2 * nu + u] = a0s; tab[3 * nu + u] = sb0; tab[4 * nu + u] = lsp;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Per-pixel, per-view footprint enumeration                                  */
+/* Each basis enumerates (bin, weight) with weight such that                  */
+/*   A: sino[p][u] += weight * img[pix]       A^T: img[pix] += weight * sino  */
+/* and follows the reference's skip rules (conv > 1e-7, NaN guard).           */
+/* ------------------------------------------------------------------------- */
+typedef void (*emit_fn)(void *ctx, int u, double w);
+
+/* B1 for one (pixel, view): layers/A_B1.cu:62-208 (forward) / :211-353. */
+static void b1_pixel(const ogeo *g, int p, const fpt *vx, const fpt *vy, int ix, int iy, int pad,
+                     emit_fn emit, void *ctx) {
+    int nx = g->nx, ny = g->ny, nu = g->nu;
+    fpt dx = (fpt)g->dx, dy = (fpt)g->dy;
+    fpt px = (fpt)g->src[2 * p], py = (fpt)g->src[2 * p + 1];
+    fpt pux = (fpt)g->puv[2 * p], puy = (fpt)g->puv[2 * p + 1];
+    fpt dtx = (fpt)g->dtv[2 * p], dty = (fpt)g->dtv[2 * p + 1];
+    fpt us[4];
+    int minu, maxu;
+    fpt ox = ((fpt)ix - (fpt)0.5f * (fpt)(nx - 1)) * dx;
+    fpt oy = ((fpt)iy - (fpt)0.5f * (fpt)(ny - 1)) * dy;
+    corner_range(g->pmi + 12 * p, ix, iy, (fpt)1.0f, nu, us, &minu, &maxu);
+    minu = imax(0, minu - pad);
+    maxu = imin(nu - 1, maxu + pad);
+    fpt sx = dtx + pux * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    fpt sy = dty + puy * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    for (int u = minu; u <= maxu; ++u, sx += pux, sy += puy) {
+        fpt ax = sx - pux * (fpt)0.5f, ay = sy - puy * (fpt)0.5f;
+        fpt bx = sx + pux * (fpt)0.5f, by = sy + puy * (fpt)0.5f;
+        fpt ra = ((ox - px) * (sx - px) + (oy - py) * (sy - py)) / ((ax - px) * (sx - px) + (ay - py) * (sy - py));
+        fpt rb = ((ox - px) * (sx - px) + (oy - py) * (sy - py)) / ((bx - px) * (sx - px) + (by - py) * (sy - py));
+        fpt ax1 = px + (ax - px) * ra, ay1 = py + (ay - py) * ra;
+        fpt bx1 = px + (bx - px) * rb, by1 = py + (by - py) * rb;
+        fpt x1 = (fpt)sqrt((double)((ax1 - ox) * (ax1 - ox) + (ay1 - oy) * (ay1 - oy)));
+        fpt x2 = (fpt)sqrt((double)((bx1 - ox) * (bx1 - ox) + (by1 - oy) * (by1 - oy)));
+        fpt sgn = ((ax1 - ox) * (bx1 - ox) + (ay1 - oy) * (by1 - oy) > 0) ? (fpt)1.0f : (fpt)-1.0f;
+        fpt a = fp_max(vx[u], vy[u]), b = fp_min(vx[u], vy[u]);
+        fpt conv = b1_conv(x1 * sgn, x2, a, b);
+        if (conv == conv && conv > EPS7) emit(ctx, u, (double)conv * (double)dx * (double)dy);
+    }
+}
+
+/* B0 for one (pixel, view): layers/A_B0.cu:123-229 / :231-336. */
+static void b0_pixel(const ogeo *g, int p, const double *tab, int ix, int iy, int pad,
+                     emit_fn emit, void *ctx) {
+    int nx = g->nx, ny = g->ny, nu = g->nu;
+    fpt dx = (fpt)g->dx, dy = (fpt)g->dy;
+    fpt nx2 = (fpt)0.5f * (fpt)(nx - 1), ny2 = (fpt)0.5f * (fpt)(ny - 1);
+    fpt px = (fpt)g->src[2 * p], py = (fpt)g->src[2 * p + 1];
+    fpt pux = (fpt)g->puv[2 * p], puy = (fpt)g->puv[2 * p + 1];
+    fpt dtx = (fpt)g->dtv[2 * p], dty = (fpt)g->dtv[2 * p + 1];
+    fpt us[4];
+    int minu, maxu;
+    fpt ox = ((fpt)ix - nx2) * dx;
+    fpt oy = ((fpt)iy - ny2) * dy;
+    corner_range(g->pmi + 12 * p, ix, iy, (fpt)0.5f, nu, us, &minu, &maxu);
+    minu = imax(0, minu - pad);
+    maxu = imin(nu - 1, maxu + pad);
+    fpt sx = dtx + pux * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    fpt sy = dty + puy * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    for (int u = minu; u <= maxu; ++u, sx += pux, sy += puy) {
+        double vx = tab[u], vy = tab[nu + u], a0s = tab[2 * nu + u], sb0 = tab[3 * nu + u], lsp = tab[4 * nu + u];
+        double r1 = (double)((ox - px) * (sx - px) + (oy - py) * (sy - py)) / (lsp * lsp);
+        double vblur = (a0s + sb0) * (-r1);
+        fpt zx = ox - (fpt)0.5f * dx - px, zy = oy + (fpt)0.5f * dy - py;
+        double y = (double)((sx - px) * zy - zx * (sy - py)) / lsp;
+        y += a0s * r1;
+        fpt conv = b0_conv((fpt)(-y), (fpt)vx, (fpt)vy, (fpt)vblur);
+        if (conv == conv && conv > EPS7) emit(ctx, u, (double)conv * (double)dx * (double)dy);
+    }
+}
+
+/* SF trapezoid/bin overlap: layers/A_SF.cu:48-84. */
+static fpt sf_gamma(fpt s1, fpt s2, const fpt *us) {
+    fpt acc = (fpt)0.0f, tmp, b1, b2;
+    b1 = fp_max(s1, us[0]); b2 = fp_min(s2, us[1]);
+    tmp = (b2 > b1) ? ((fpt)1 / (2 * (us[1] - us[0]))) * ((b2 - us[0]) * (b2 - us[0]) - (b1 - us[0]) * (b1 - us[0])) : (fpt)0.0f;
+    if (tmp == tmp) acc += tmp;
+    b1 = fp_max(s1, us[1]); b2 = fp_min(s2, us[2]);
+    tmp = (b2 > b1) ? (b2 - b1) : (fpt)0.0f;
+    if (tmp == tmp) acc += tmp;
+    b1 = fp_max(s1, us[2]); b2 = fp_min(s2, us[3]);
+    tmp = (b2 > b1) ? ((fpt)1 / (2 * (us[3] - us[2]))) * ((b1 - us[3]) * (b1 - us[3]) - (b2 - us[3]) * (b2 - us[3])) : (fpt)0.0f;
+    if (tmp == tmp) acc += tmp;
+    return acc;
+}
+
+/* SF for one (pixel, view): layers/A_SF.cu:87-225 / :228-334. */
+static void sf_pixel(const ogeo *g, int p, int ix, int iy, int pad, emit_fn emit, void *ctx) {
+    int nx = g->nx, ny = g->ny, nu = g->nu;
+    fpt dx = (fpt)g->dx, dy = (fpt)g->dy;
+    fpt nx2 = (fpt)(nx - 1) / (fpt)2.0f, ny2 = (fpt)(ny - 1) / (fpt)2.0f;
+    fpt px = (fpt)g->src[2 * p], py = (fpt)g->src[2 * p + 1];
+    fpt us[4];
+    int minu, maxu;
+    corner_range(g->pmi + 12 * p, ix, iy, (fpt)0.5f, nu, us, &minu, &maxu);
+    minu = imax(0, minu - pad);
+    maxu = imin(nu - 1, maxu + pad);
+    fpt th = (fpt)atan2((double)(dx * ((fpt)ix - nx2) - px), (double)(dy * ((fpt)iy - ny2) - py));
+    fpt st = fp_abs((fpt)sin((double)th)), ct = fp_abs((fpt)cos((double)th));
+    fpt weight = dx / fp_max(st, ct);
+    for (int u = minu; u <= maxu; ++u) {
+        fpt gm = sf_gamma((fpt)u, (fpt)u + (fpt)1.0f, us);
+        double w = (double)weight * (double)gm;
+        if (w == w && gm > EPS7) emit(ctx, u, w);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Whole-image drivers (OpenMP: A over views, A^T over pixels).              */
+/* ------------------------------------------------------------------------- */
+typedef struct { double *row; double c; } fwd_ctx;
+static void emit_fwd(void *ctx, int u, double w) {
+    fwd_ctx *f = (fwd_ctx *)ctx;
+    f->row[u] += w * f->c;
+}
+typedef struct { const double *row; double acc; } bwd_ctx;
+static void emit_bwd(void *ctx, int u, double w) {
+    bwd_ctx *b = (bwd_ctx *)ctx;
+    b->acc += w * b->row[u];
+}
+typedef struct { int *u; double *w; int n, cap; } list_ctx;
+static void emit_list(void *ctx, int u, double w) {
+    list_ctx *l = (list_ctx *)ctx;
+    if (l->n < l->cap) { l->u[l->n] = u; l->w[l->n] = w; }
+    l->n++;
+}
+
+enum { OR_B0 = 0, OR_B1 = 1, OR_SF = 2 };
+
+static void pixel_dispatch(int basis, const ogeo *g, int p, const void *tab, int ix, int iy, int pad,
+                           emit_fn e, void *ctx) {
+    if (basis == OR_B1) {
+        const fpt *vx = (const fpt *)tab;
+        b1_pixel(g, p, vx, vx + g->nu, ix, iy, pad, e, ctx);
+    } else if (basis == OR_B0) {
+        b0_pixel(g, p, (const double *)tab, ix, iy, pad, e, ctx);
+    } else {
+        sf_pixel(g, p, ix, iy, pad, e, ctx);
+    }
+}
+
+static void *make_tab(int basis, const ogeo *g, int p) {
+    if (basis == OR_B1) {
+        fpt *t = (fpt *)malloc(sizeof(fpt) * 2 * g->nu);
+        b1_vectors(g, p, t, t + g->nu);
+        return t;
+    }
+    if (basis == OR_B0) {
+        double *t = (double *)malloc(sizeof(double) * 5 * g->nu);
+        b0_vectors(g, p, t);
+        return t;
+    }
+    return NULL;
+}
+
+/* A: sino[p*nu+u] = sum over pixels; views [p0, p1) only (others untouched). */
+void or_forward(int basis, const void *h, const double *img, double *sino, int p0, int p1, int pad) {
+    const ogeo *g = (const ogeo *)h;
+    int nx = g->nx, ny = g->ny, nu = g->nu;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = p0; p < p1; ++p) {
+        void *tab = make_tab(basis, g, p);
+        double *row = sino + (size_t)p * nu;
+        memset(row, 0, sizeof(double) * nu);
+        fwd_ctx f;
+        f.row = row;
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                f.c = img[(size_t)iy * nx + ix];
+                pixel_dispatch(basis, g, p, tab, ix, iy, pad, emit_fwd, &f);
+            }
+        free(tab);
+    }
+}
+
+/* A^T over views [p0, p1): img[pix] = sum_p sum_u w * sino[p][u]. */
+void or_backward(int basis, const void *h, const double *sino, double *img, int p0, int p1, int pad) {
+    const ogeo *g = (const ogeo *)h;
+    int nx = g->nx, ny = g->ny, nu = g->nu;
+    int np = p1 - p0;
+    void **tabs = (void **)malloc(sizeof(void *) * (np > 0 ? np : 1));
+    for (int k = 0; k < np; ++k) tabs[k] = make_tab(basis, g, p0 + k);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int iy = 0; iy < ny; ++iy) {
+        for (int ix = 0; ix < nx; ++ix) {
+            double acc = 0.0;
+            for (int k = 0; k < np; ++k) {
+                bwd_ctx b;
+                b.row = sino + (size_t)(p0 + k) * nu;
+                b.acc = 0.0;
+                pixel_dispatch(basis, g, p0 + k, tabs[k], ix, iy, pad, emit_bwd, &b);
+                acc += b.acc;
+            }
+            img[(size_t)iy * nx + ix] = acc;
+        }
+    }
+    for (int k = 0; k < np; ++k) free(tabs[k]);
+    free(tabs);
+}
+
+/* Footprint of one pixel in one view: returns the number of (bin, weight)
+ * pairs (written up to cap). */
+int or_footprint(int basis, const void *h, int p, int ix, int iy, int pad, int *u_out, double *w_out, int cap) {
+    const ogeo *g = (const ogeo *)h;
+    void *tab = make_tab(basis, g, p);
+    list_ctx l = {u_out, w_out, 0, cap};
+    pixel_dispatch(basis, g, p, tab, ix, iy, pad, emit_list, &l);
+    free(tab);
+    return l.n;
+}
+
+/* Number of footprint evaluations (sum of corner-range lengths) of views
+ * [p0, p1): the reference's conv_count (layers/A_B1.cu:50) without skips. */
+long long or_eval_count(int basis, const void *h, int p0, int p1) {
+    const ogeo *g = (const ogeo *)h;
+    fpt hw = (basis == OR_B1) ? (fpt)1.0f : (fpt)0.5f;
+    long long n = 0;
+#pragma omp parallel for reduction(+ : n)
+    for (int p = p0; p < p1; ++p)
+        for (int iy = 0; iy < g->ny; ++iy)
+            for (int ix = 0; ix < g->nx; ++ix) {
+                fpt us[4];
+                int a, b;
+                corner_range(g->pmi + 12 * p, ix, iy, hw, g->nu, us, &a, &b);
+                n += b - a + 1;
+            }
+    return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP64 exact strip integrals (independent of the footprint model).           */
+/* value(pixel, bin) = average over the detector bin (uniform in detector     */
+/* coordinate) of the line integral, along the ray source -> detector point, */
+/* of the basis function: the pixel indicator (B0) or the bilinear hat (B1). */
+/* ------------------------------------------------------------------------- */
+
+/* chord length of the line through P with unit direction (ex, ey) inside the
+ * box [x0,x1]x[y0,y1] */
+static double chord_box(double Px, double Py, double ex, double ey, double x0, double x1, double y0,
+                        double y1) {
+    double tmin = -1e300, tmax = 1e300;
+    if (fabs(ex) < 1e-300) {
+        if (Px < x0 || Px > x1) return 0.0;
+    } else {
+        double ta = (x0 - Px) / ex, tb = (x1 - Px) / ex;
+        if (ta > tb) { double t = ta; ta = tb; tb = t; }
+        if (ta > tmin) tmin = ta;
+        if (tb < tmax) tmax = tb;
+    }
+    if (fabs(ey) < 1e-300) {
+        if (Py < y0 || Py > y1) return 0.0;
+    } else {
+        double ta = (y0 - Py) / ey, tb = (y1 - Py) / ey;
+        if (ta > tb) { double t = ta; ta = tb; tb = t; }
+        if (ta > tmin) tmin = ta;
+        if (tb < tmax) tmax = tb;
+    }
+    return (tmax > tmin) ? (tmax - tmin) : 0.0;
+}
+
+/* line integral of hat(x)hat(y), hat(t) = max(0, 1 - |t|/h), centred at
+ * (cx, cy), along the line P + s e (unit e): piecewise quadratic in s, split
+ * at the crossings of x = cx + {-dx, 0, dx} and y = cy + {-dy, 0, dy} and
+ * integrated exactly by Simpson's rule on every piece. */
+static double hat_line(double Px, double Py, double ex, double ey, double cx, double cy, double dx,
+                       double dy) {
+    double br[8];
+    int nb = 0;
+    double lo = -1e300, hi = 1e300;
+    /* support of the product: |x-cx| < dx and |y-cy| < dy */
+    if (fabs(ex) < 1e-300) {
+        if (fabs(Px - cx) >= dx) return 0.0;
+    } else {
+        double ta = (cx - dx - Px) / ex, tb = (cx + dx - Px) / ex;
+        if (ta > tb) { double t = ta; ta = tb; tb = t; }
+        if (ta > lo) lo = ta;
+        if (tb < hi) hi = tb;
+        br[nb++] = (cx - Px) / ex;
+    }
+    if (fabs(ey) < 1e-300) {
+        if (fabs(Py - cy) >= dy) return 0.0;
+    } else {
+        double ta = (cy - dy - Py) / ey, tb = (cy + dy - Py) / ey;
+        if (ta > tb) { double t = ta; ta = tb; tb = t; }
+        if (ta > lo) lo = ta;
+        if (tb < hi) hi = tb;
+        br[nb++] = (cy - Py) / ey;
+    }
+    if (!(hi > lo)) return 0.0;
+    double pts[4];
+    int np_ = 0;
+    pts[np_++] = lo;
+    for (int i = 0; i < nb; ++i)
+        if (br[i] > lo && br[i] < hi) pts[np_++] = br[i];
+    pts[np_++] = hi;
+    /* sort (at most 4 points) */
+    for (int i = 1; i < np_; ++i)
+        for (int j = i; j > 0 && pts[j] < pts[j - 1]; --j) { double t = pts[j]; pts[j] = pts[j - 1]; pts[j - 1] = t; }
+    double sum = 0.0;
+    for (int i = 0; i + 1 < np_; ++i) {
+        double s0 = pts[i], s1 = pts[i + 1], sm = 0.5 * (s0 + s1);
+        double f[3];
+        double ss[3] = {s0, sm, s1};
+        for (int k = 0; k < 3; ++k) {
+            double x = Px + ss[k] * ex, y = Py + ss[k] * ey;
+            double hx = 1.0 - fabs(x - cx) / dx, hy = 1.0 - fabs(y - cy) / dy;
+            f[k] = (hx > 0 ? hx : 0) * (hy > 0 ? hy : 0);
+        }
+        sum += (s1 - s0) * (f[0] + 4 * f[1] + f[2]) / 6.0;
+    }
+    return sum;
+}
+
+static double ray_value(int basis, double px, double py, double qx, double qy, double cx, double cy,
+                        double dx, double dy) {
+    double ex = qx - px, ey = qy - py;
+    double n = sqrt(ex * ex + ey * ey);
+    ex /= n;
+    ey /= n;
+    if (basis == OR_B1) return hat_line(px, py, ex, ey, cx, cy, dx, dy);
+    return chord_box(px, py, ex, ey, cx - 0.5 * dx, cx + 0.5 * dx, cy - 0.5 * dy, cy + 0.5 * dy);
+}
+
+/* detector parameter tau in [0,1] across bin u of the ray through (X, Y) */
+static double tau_of(const ogeo *g, int p, int u, double X, double Y) {
+    double uu = or_geo_u(g, p, X, Y);
+    return uu - u;
+}
+
+static const double GL8x[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
+                               -0.1834346424956498, 0.1834346424956498,  0.5255324099163290,
+                               0.7966664774136267,  0.9602898564975363};
+static const double GL8w[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066661925261,
+                               0.3626837833783620, 0.3626837833783620, 0.3137066661925261,
+                               0.2223810344533745, 0.1012285362903763};
+
+/* Exact average over bin u of view p of the basis line integral for pixel
+ * (ix, iy), unit coefficient.  Gauss-Legendre (8 points) on the sub-intervals
+ * of the bin between the rays through the basis function's kinks. */
+double or_strip_exact(int basis, const void *h, int p, int u, int ix, int iy) {
+    const ogeo *g = (const ogeo *)h;
+    double dx = g->dx, dy = g->dy;
+    double cx = (ix - 0.5 * (g->nx - 1)) * dx, cy = (iy - 0.5 * (g->ny - 1)) * dy;
+    double px = g->src[2 * p], py = g->src[2 * p + 1];
+    double pux = g->puv[2 * p], puy = g->puv[2 * p + 1];
+    double ax = g->dtv[2 * p] + pux * (u - 0.5 * g->nu), ay = g->dtv[2 * p + 1] + puy * (u - 0.5 * g->nu);
+    double cuts[16];
+    int nc = 0;
+    cuts[nc++] = 0.0;
+    double hx = (basis == OR_B1) ? dx : 0.5 * dx, hy = (basis == OR_B1) ? dy : 0.5 * dy;
+    for (int i = -1; i <= 1; ++i)
+        for (int j = -1; j <= 1; ++j) {
+            if (basis != OR_B1 && (i == 0 || j == 0)) continue;
+            double t = tau_of(g, p, u, cx + i * hx, cy + j * hy);
+            if (t > 0 && t < 1) cuts[nc++] = t;
+        }
+    cuts[nc++] = 1.0;
+    for (int i = 1; i < nc; ++i)
+        for (int j = i; j > 0 && cuts[j] < cuts[j - 1]; --j) { double t = cuts[j]; cuts[j] = cuts[j - 1]; cuts[j - 1] = t; }
+    double total = 0.0;
+    for (int k = 0; k + 1 < nc; ++k) {
+        double t0 = cuts[k], t1 = cuts[k + 1];
+        if (t1 <= t0) continue;
+        double half = 0.5 * (t1 - t0), mid = 0.5 * (t1 + t0);
+        for (int q = 0; q < 8; ++q) {
+            double t = mid + half * GL8x[q];
+            double qx = ax + pux * t, qy = ay + puy * t;
+            total += half * GL8w[q] * ray_value(basis, px, py, qx, qy, cx, cy, dx, dy);
+        }
+    }
+    return total;
+}
+
+/* Adaptive Simpson over the bin with Richardson correction (q + (q-v)/15),
+ * the 1-D analogue of layers/Ref_projector.cu:156-214; returns the average
+ * over the bin and writes the accumulated |q-v|/15 error estimate. */
+static double simpson_rec(int basis, const ogeo *g, int p, double ax, double ay, double pux,
+                          double puy, double cx, double cy, double t0, double t1, double f0,
+                          double fm, double f1, double whole, double eps, int depth, double *err) {
+    double dx = g->dx, dy = g->dy, px = g->src[2 * p], py = g->src[2 * p + 1];
+    double tm = 0.5 * (t0 + t1);
+    double tl = 0.5 * (t0 + tm), tr = 0.5 * (tm + t1);
+    double fl = ray_value(basis, px, py, ax + pux * tl, ay + puy * tl, cx, cy, dx, dy);
+    double fr = ray_value(basis, px, py, ax + pux * tr, ay + puy * tr, cx, cy, dx, dy);
+    double left = (tm - t0) * (f0 + 4 * fl + fm) / 6.0;
+    double right = (t1 - tm) * (fm + 4 * fr + f1) / 6.0;
+    double q = left + right;
+    if (depth <= 0 || fabs(q - whole) <= 15.0 * eps) {
+        *err += fabs(q - whole) / 15.0;
+        return q + (q - whole) / 15.0;
+    }
+    return simpson_rec(basis, g, p, ax, ay, pux, puy, cx, cy, t0, tm, f0, fl, fm, left, 0.5 * eps, depth - 1, err) +
+           simpson_rec(basis, g, p, ax, ay, pux, puy, cx, cy, tm, t1, fm, fr, f1, right, 0.5 * eps, depth - 1, err);
+}
+
+double or_strip_simpson(int basis, const void *h, int p, int u, int ix, int iy, double eps, int depth,
+                        double *err_out) {
+    const ogeo *g = (const ogeo *)h;
+    double dx = g->dx, dy = g->dy;
+    double cx = (ix - 0.5 * (g->nx - 1)) * dx, cy = (iy - 0.5 * (g->ny - 1)) * dy;
+    double px = g->src[2 * p], py = g->src[2 * p + 1];
+    double pux = g->puv[2 * p], puy = g->puv[2 * p + 1];
+    double ax = g->dtv[2 * p] + pux * (u - 0.5 * g->nu), ay = g->dtv[2 * p + 1] + puy * (u - 0.5 * g->nu);
+    double f0 = ray_value(basis, px, py, ax, ay, cx, cy, dx, dy);
+    double f1 = ray_value(basis, px, py, ax + pux, ay + puy, cx, cy, dx, dy);
+    double fm = ray_value(basis, px, py, ax + 0.5 * pux, ay + 0.5 * puy, cx, cy, dx, dy);
+    double whole = (f0 + 4 * fm + f1) / 6.0;
+    double err = 0.0;
+    double v = simpson_rec(basis, g, p, ax, ay, pux, puy, cx, cy, 0.0, 1.0, f0, fm, f1, whole, eps, depth, &err);
+    if (err_out) *err_out = err;
+    return v;
+}
+
+/* Exact strip-integral A for a whole image over views [p0, p1), evaluated
+ * over bins within the footprint's corner range widened by `pad`. */
+void or_forward_exact(int basis, const void *h, const double *img, double *sino, int p0, int p1, int pad) {
+    const ogeo *g = (const ogeo *)h;
+    int nx = g->nx, ny = g->ny, nu = g->nu;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = p0; p < p1; ++p) {
+        double *row = sino + (size_t)p * nu;
+        memset(row, 0, sizeof(double) * nu);
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                double c = img[(size_t)iy * nx + ix];
+                if (c == 0.0) continue;
+                fpt us[4];
+                int a, b;
+                corner_range(g->pmi + 12 * p, ix, iy, (basis == OR_B1) ? (fpt)1.0f : (fpt)0.5f, nu, us, &a, &b);
+                a = imax(0, a - pad);
+                b = imin(nu - 1, b + pad);
+                for (int u = a; u <= b; ++u) row[u] += c * or_strip_exact(basis, g, p, u, ix, iy);
+            }
+    }
+}
+
+int or_precision_bits(void) { return (int)(sizeof(fpt) * 8); }

@@ -1,0 +1,81 @@
+"""ctypes binding of sgd_amd/libsgdct.so (declared in include/sgdct.h).
+
+There is no CPU fallback: every projector call goes to the gfx950 kernels,
+and a missing library or a failed call raises.
+"""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsgdct.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "sgdct.h")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.c_void_p
+_h = ctypes.c_void_p
+_lib = None
+
+
+class SgdctError(RuntimeError):
+    pass
+
+
+def header_functions(path=HEADER):
+    """Names of the functions include/sgdct.h declares."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "defined"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def lib():
+    """Load the library (raises SgdctError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SgdctError("%s is missing: build it with `python -m sgd_amd.build` "
+                         "(or __graft_entry__.build())" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    init12 = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [_dp, _dp, _dp]
+    for name in ("b0_init", "b1_init"):
+        f = getattr(L, name)
+        f.argtypes = init12
+        f.restype = _h
+    L.SF_init.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [_dp]
+    L.SF_init.restype = _h
+    for pre in ("b0", "b1", "SF"):
+        for kind in ("forward", "backward"):
+            f = getattr(L, "%s_%s_projection" % (pre, kind))
+            f.argtypes = [_dp, _dp, _h]
+            f.restype = ctypes.c_int
+    L.sgdct_destroy.argtypes = [_h]
+    L.sgdct_destroy.restype = None
+    for name in ("sgdct_forward_dev", "sgdct_backward_dev"):
+        f = getattr(L, name)
+        f.argtypes = [_h, _fp, _fp, ctypes.c_int, _fp]
+        f.restype = ctypes.c_int
+    L.sgdct_shape.argtypes = [_h, ctypes.POINTER(ctypes.c_int)]
+    L.sgdct_shape.restype = ctypes.c_int
+    L.sgdct_eval_count.argtypes = [_h]
+    L.sgdct_eval_count.restype = ctypes.c_longlong
+    L.sgdct_last_error.argtypes = []
+    L.sgdct_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().sgdct_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc < 0:
+        raise SgdctError("%s failed (%d): %s" % (what, rc, last_error()))
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)

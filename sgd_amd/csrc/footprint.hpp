@@ -1,0 +1,79 @@
+// footprint.hpp -- 1-D box-spline footprints evaluated on the GPU (FP32).
+//
+// Every 2-D fan-beam basis here reduces, for one (pixel, bin) pair, to the
+// average over an interval [ta, tb] of a unit-integral 1-D kernel K:
+//     conv = (S(tb) - S(ta)) / (tb - ta),   S the odd antiderivative of K,
+// where ta < tb are the signed offsets, from the pixel centre, at which the two
+// edge rays of the bin cross the line through the pixel centre perpendicular to
+// the bin's central ray.  This is exactly the quantity of
+//   layers/A_B1.cu:182-197 + spline_conv.hh:372-390 (fast_2tri_1box_spline / |x2-x1|)
+//   layers/A_B0.cu:211-219 + spline_conv.hh:244-293 (3-direction box spline)
+// written in a form without the reference's catastrophic cancellations.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sgdct {
+
+// Per-bin constants of the B1 kernel K = tri_a * tri_b (unit-integral hats
+// of half-widths a >= b; layers/A_B1.cu:462-463, 197).  With y = a + b - |t|:
+//   S(t) = sign(t) * (1/2 - c*T(y) + 2c*max(y - a, 0)^4),  c = 1/(24 a^2 b^2)
+//   T(y) = y^4 - 2(y-b)_+^4 + (y-2b)_+^4
+//        = 12 b^2 (y-b)^2 + 2 b^4           for y >= 2b (expanded: no cancellation)
+// For b <= 1e-7 the reference switches to tri_a * box (spline_conv.hh:393-405),
+// which is the b -> 0 limit of the same expression (c terms dropped).
+struct B1Bin {
+  float ka, kb;      // ta = ka * N_u(pixel), tb = kb * N_{u+1}(pixel)
+  float apb, b, a;   // a + b, b, a
+  float k2, k4;      // c*12 b^2 = 1/(2a^2),  c*2 b^4 = b^2/(12 a^2)
+  float cq, c2;      // c, 2c (0 when b <= 1e-7)
+  float pad0, pad1, pad2;
+};
+
+// Per-bin constants of the B0 kernel K = box_p * box_q (trapezoid, p >= q;
+// p, q = |vx|, |vy| of layers/A_B0.cu:369-373).  S(t) for x = |t|:
+//   x <= (p-q)/2 :  x / p
+//   otherwise    :  1/2 - max((p+q)/2 - x, 0)^2 / (2 p q)
+struct B0Bin {
+  float ka, kb;
+  float hpq, hmq;    // (p+q)/2, (p-q)/2
+  float inv_p, inv_2pq;
+  float pad0, pad1;
+};
+
+__device__ __forceinline__ float copysign_f(float mag, float sgn) {
+  return __builtin_copysignf(mag, sgn);
+}
+
+__device__ __forceinline__ float s_b1(float t, float apb, float b, float a, float k2, float k4,
+                                      float cq, float c2) {
+  float x = __builtin_fabsf(t);
+  float y = __builtin_fmaxf(apb - x, 0.0f);
+  float v = y - b;
+  float qa = __builtin_fmaf(k2, v * v, k4);
+  float y2 = y * y;
+  float m = __builtin_fmaxf(v, 0.0f);
+  float m2 = m * m;
+  float qb = cq * __builtin_fmaf(-2.0f, m2 * m2, y2 * y2);
+  float cqv = (v >= b) ? qa : qb;
+  float e = __builtin_fmaxf(y - a, 0.0f);
+  float e2 = e * e;
+  float s = __builtin_fmaf(c2, e2 * e2, 0.5f - cqv);
+  return copysign_f(s, t);
+}
+
+__device__ __forceinline__ float s_b0(float t, float hpq, float hmq, float inv_p, float inv_2pq) {
+  float x = __builtin_fabsf(t);
+  float lo = x * inv_p;
+  float r = __builtin_fmaxf(hpq - x, 0.0f);
+  float hi = __builtin_fmaf(-r * r, inv_2pq, 0.5f);
+  return copysign_f((x <= hmq) ? lo : hi, t);
+}
+
+// Average of K over [ta, tb] (ta < tb), with the reference's skip rule
+// (values <= 1e-7 and NaN dropped: layers/A_B1.cu:203, A_B0.cu:225).
+__device__ __forceinline__ float mean_over(float sa, float sb, float ta, float tb) {
+  float conv = (sb - sa) * __builtin_amdgcn_rcpf(tb - ta);
+  return (conv > 1e-7f) ? conv : 0.0f;
+}
+
+}  // namespace sgdct

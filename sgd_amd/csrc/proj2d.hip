@@ -1,0 +1,308 @@
+// proj2d.hip -- gfx950 kernels of the 2-D fan-beam footprint projectors.
+//
+//   fwd_bin_kernel   A   (image -> sinogram).  One lane per (view, bin): the
+//                    lane walks the pixels of its bin's strip line by line and
+//                    gathers sum_pixel conv * x.  No atomics, deterministic.
+//   bwd_pix_kernel   A^T (sinogram -> image).  One lane per pixel, 16x16
+//                    pixel tiles per workgroup; per-bin constants and the
+//                    sinogram values of the bins the tile sees are staged in
+//                    LDS, kViewChunk views at a time.  Deterministic gather.
+//   count_kernel     number of footprint evaluations under the reference's
+//                    corner rule (the value layers/A_B1.cu:50,643 returns).
+//
+// Footprint values follow layers/A_B1.cu:62-208 (B1), layers/A_B0.cu:123-229
+// (B0), layers/A_SF.cu:48-225 (SF); see footprint.hpp for the form used.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "footprint.hpp"
+#include "projector.hpp"
+
+namespace sgdct {
+
+// ---------------------------------------------------------------------------
+// Basis traits: per-bin record, footprint of one (pixel, bin) pair.
+// ---------------------------------------------------------------------------
+template <int BASIS> struct Traits;
+
+template <> struct Traits<BASIS_B1> {
+  using Bin = B1Bin;
+  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
+    float sa = s_b1(ta, r.apb, r.b, r.a, r.k2, r.k4, r.cq, r.c2);
+    float sb = s_b1(tb, r.apb, r.b, r.a, r.k2, r.k4, r.cq, r.c2);
+    return mean_over(sa, sb, ta, tb);
+  }
+};
+
+template <> struct Traits<BASIS_B0> {
+  using Bin = B0Bin;
+  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
+    float sa = s_b0(ta, r.hpq, r.hmq, r.inv_p, r.inv_2pq);
+    float sb = s_b0(tb, r.hpq, r.hmq, r.inv_p, r.inv_2pq);
+    return mean_over(sa, sb, ta, tb);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// A: bin-driven gather.
+// ---------------------------------------------------------------------------
+template <int BASIS>
+__global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float* __restrict__ img,
+                                                      float* __restrict__ sino) {
+  using T = Traits<BASIS>;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int v = wave / g.nchunk;
+  if (v >= g.np) return;
+  const int chunk = wave - v * g.nchunk;
+  const int u = chunk * 64 + lane;
+  img += (size_t)blockIdx.y * g.nx * g.ny;
+  sino += (size_t)blockIdx.y * g.np * g.nu;
+
+  const ViewGeo vg = g.views[v];
+  // Walk rows when the wave's middle ray is closer to the y axis, columns
+  // otherwise (wave-uniform choice).
+  const int umid = min(chunk * 64 + 32, g.nu - 1);
+  const double mx = vg.e0x + (umid + 0.5) * vg.pux, my = vg.e0y + (umid + 0.5) * vg.puy;
+  const bool rows = fabs(my) >= fabs(mx);
+  if (u >= g.nu) return;
+
+  const typename T::Bin rec = reinterpret_cast<const typename T::Bin*>(g.bins)[(size_t)v * g.nu + u];
+  // N_u and N_{u+1} as affine functions of the pixel indices.
+  const double eax = vg.e0x + u * vg.pux, eay = vg.e0y + u * vg.puy;
+  const double ebx = eax + vg.pux, eby = eay + vg.puy;
+  const double acx = -eay * g.dx, acy = eax * g.dy, ac0 = eax * vg.w0y - eay * vg.w0x;
+  const double bcx = -eby * g.dx, bcy = ebx * g.dy, bc0 = ebx * vg.w0y - eby * vg.w0x;
+  // Corner rule: bin u is in a pixel's range iff N_u + R_u >= 0 and
+  // N_{u+1} - R_{u+1} < 0 (max / min of N over the pixel's corners).
+  const double ra = g.h * (fabs(acx) + fabs(acy)), rb = g.h * (fabs(bcx) + fabs(bcy));
+  const double sa = rows ? acx : acy, ma = rows ? acy : acx;
+  const double sb = rows ? bcx : bcy, mb = rows ? bcy : bcx;
+  const int nmin = rows ? g.nx : g.ny, nmaj = rows ? g.ny : g.nx;
+  const size_t smin = rows ? 1 : g.nx, smaj = rows ? g.nx : 1;
+  const double isa = sa != 0.0 ? 1.0 / sa : 0.0, isb = sb != 0.0 ? 1.0 / sb : 0.0;
+  const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
+
+  // Lines crossed by the strip: between the extreme crossings of its two
+  // boundary lines with the first and last pixel of a line.
+  int m_lo = 0, m_hi = nmaj - 1;
+  if (ma != 0.0 && mb != 0.0) {
+    const double last = nmin - 1;
+    double c1 = (-ra - ac0) / ma, c2 = (-ra - ac0 - sa * last) / ma;
+    double c3 = (rb - bc0) / mb, c4 = (rb - bc0 - sb * last) / mb;
+    double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
+    lo = fmax(floor(lo) - 1.0, 0.0);
+    hi = fmin(ceil(hi) + 1.0, (double)(nmaj - 1));
+    m_lo = (int)lo;
+    m_hi = (int)hi;
+  }
+
+  double acc = 0.0;
+  for (int m = m_lo; m <= m_hi; ++m) {
+    const double r1 = -ra - ma * m - ac0;  // need sa * n >= r1
+    const double r2 = rb - mb * m - bc0;   // need sb * n <  r2
+    double lo = 0.0, hi = (double)(nmin - 1);
+    if (sa > 0.0) lo = fmax(lo, ceil(r1 * isa));
+    else if (sa < 0.0) hi = fmin(hi, floor(r1 * isa));
+    else if (r1 > 0.0) continue;
+    if (sb > 0.0) hi = fmin(hi, ceil(r2 * isb) - 1.0);
+    else if (sb < 0.0) lo = fmax(lo, floor(r2 * isb) + 1.0);
+    else if (r2 <= 0.0) continue;
+    if (lo > hi) continue;
+    const int n0 = (int)lo, n1 = (int)hi;
+    float ta = (float)((double)rec.ka * (sa * n0 + ma * m + ac0));
+    float tb = (float)((double)rec.kb * (sb * n0 + mb * m + bc0));
+    const float* p = img + (size_t)m * smaj + (size_t)n0 * smin;
+    float accl = 0.0f;
+    for (int n = n0; n <= n1; ++n) {
+      const float c = T::conv(rec, ta, tb);
+      accl = __builtin_fmaf(c, *p, accl);
+      p += smin;
+      ta += dta;
+      tb += dtb;
+    }
+    acc += (double)accl;
+  }
+  sino[(size_t)v * g.nu + u] = (float)(acc * g.dx * g.dy);
+}
+
+// ---------------------------------------------------------------------------
+// A^T: pixel-driven gather.
+// ---------------------------------------------------------------------------
+
+// Bin window of one view for a pixel tile: bins touched by the tile's box
+// widened by h, from the exact projection of its four corners, +-1 margin.
+__device__ inline void tile_window(const Proj2DArgs& g, const ViewGeo& vg, int ix0, int iy0, int* lo,
+                                   int* n) {
+  double umin = 1e300, umax = -1e300;
+  for (int c = 0; c < 4; ++c) {
+    double fx = (c & 1) ? (ix0 + kTile - 1 + g.h) : (ix0 - g.h);
+    double fy = (c & 2) ? (iy0 + kTile - 1 + g.h) : (iy0 - g.h);
+    double wx = fx * g.dx + vg.w0x, wy = fy * g.dy + vg.w0y;
+    double n0 = vg.e0x * wy - vg.e0y * wx;
+    double be = vg.pux * wy - vg.puy * wx;
+    double uu = -n0 / be;
+    umin = fmin(umin, uu);
+    umax = fmax(umax, uu);
+  }
+  double l = fmax(floor(umin) - 1.0, 0.0), hgh = fmin(floor(umax) + 1.0, (double)(g.nu - 1));
+  int li = (int)l, hi = (int)hgh;
+  *lo = li;
+  *n = hi >= li ? min(hi - li + 1, kMaxWindow) : 0;
+}
+
+template <int BASIS>
+__device__ __forceinline__ typename Traits<BASIS>::Bin load_rec(const float4* r) {
+  typename Traits<BASIS>::Bin b;
+  float4* d = reinterpret_cast<float4*>(&b);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(b) / 16); ++k) d[k] = r[k];
+  return b;
+}
+
+template <int BASIS>
+__global__ __launch_bounds__(256) void bwd_pix_kernel(Proj2DArgs g, const float* __restrict__ sino,
+                                                      float* __restrict__ img) {
+  using T = Traits<BASIS>;
+  using Bin = typename T::Bin;
+  constexpr int BIN4 = sizeof(Bin) / 16;
+  static_assert(sizeof(Bin) % 16 == 0, "bin records are whole float4s");
+  __shared__ float4 lds[kViewChunk][kMaxWindow][BIN4];
+  __shared__ float ldsy[kViewChunk][kMaxWindow];
+  __shared__ int win_lo[kViewChunk], win_n[kViewChunk];
+
+  const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+  const int ix0 = blockIdx.x * kTile, iy0 = blockIdx.y * kTile;
+  const int ix = ix0 + tx, iy = iy0 + ty;
+  sino += (size_t)blockIdx.z * g.np * g.nu;
+  img += (size_t)blockIdx.z * g.nx * g.ny;
+  const bool valid = ix < g.nx && iy < g.ny;
+  const Bin* bins = reinterpret_cast<const Bin*>(g.bins);
+
+  double acc = 0.0;
+  for (int v0 = 0; v0 < g.np; v0 += kViewChunk) {
+    const int nv = min(kViewChunk, g.np - v0);
+    if ((int)threadIdx.x < nv) {
+      int lo, n;
+      tile_window(g, g.views[v0 + threadIdx.x], ix0, iy0, &lo, &n);
+      win_lo[threadIdx.x] = lo;
+      win_n[threadIdx.x] = n;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nv * kMaxWindow; idx += blockDim.x) {
+      const int j = idx / kMaxWindow, i = idx - j * kMaxWindow;
+      if (i < win_n[j]) {
+        const size_t gi = (size_t)(v0 + j) * g.nu + win_lo[j] + i;
+        const float4* src = reinterpret_cast<const float4*>(bins + gi);
+#pragma unroll
+        for (int k = 0; k < BIN4; ++k) lds[j][i][k] = src[k];
+        ldsy[j][i] = sino[gi];
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      for (int j = 0; j < nv; ++j) {
+        const int v = v0 + j;
+        const ViewGeo& vg = g.views[v];
+        const double wx = ix * g.dx + vg.w0x, wy = iy * g.dy + vg.w0y;
+        const double n0 = vg.e0x * wy - vg.e0y * wx;
+        const double be = vg.pux * wy - vg.puy * wx;
+        const float bef = (float)be;
+        const float ibe = __builtin_amdgcn_rcpf(bef);
+        const float kc = -(float)n0 * ibe;
+        const float gx = __builtin_fmaf(kc, vg.gx1, vg.gx0), gy = __builtin_fmaf(kc, vg.gy1, vg.gy0);
+        const float hw = g.h * (__builtin_fabsf(gx) + __builtin_fabsf(gy)) *
+                         __builtin_amdgcn_rcpf(__builtin_fabsf(bef) - vg.db);
+        const float flo = floorf(kc - hw), fhi = floorf(kc + hw);
+        if (fhi < 0.0f || flo > (float)(g.nu - 1)) continue;
+        const int ulo = max((int)flo, 0), uhi = min((int)fhi, g.nu - 1);
+        float na = (float)(n0 + (double)ulo * be);
+        const int wlo = win_lo[j], wn = win_n[j];
+        float accv = 0.0f;
+        for (int u = ulo; u <= uhi; ++u) {
+          const int wi = u - wlo;
+          Bin rec;
+          float yv;
+          if (wi >= 0 && wi < wn) {
+            rec = load_rec<BASIS>(&lds[j][wi][0]);
+            yv = ldsy[j][wi];
+          } else {
+            rec = bins[(size_t)v * g.nu + u];
+            yv = sino[(size_t)v * g.nu + u];
+          }
+          const float nb = na + bef;
+          const float ta = rec.ka * na, tb = rec.kb * nb;
+          accv = __builtin_fmaf(T::conv(rec, ta, tb), yv, accv);
+          na = nb;
+        }
+        acc += (double)accv;
+      }
+    }
+    __syncthreads();
+  }
+  if (valid) img[(size_t)iy * g.nx + ix] = (float)(acc * g.dx * g.dy);
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation count under the reference's clamped corner rule.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void count_kernel(Proj2DArgs g, unsigned long long* out) {
+  __shared__ unsigned long long part[256];
+  const int ix = blockIdx.x * 16 + (threadIdx.x & 15), iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+  unsigned long long c = 0;
+  if (ix < g.nx && iy < g.ny) {
+    for (int v = 0; v < g.np; ++v) {
+      const ViewGeo& vg = g.views[v];
+      double umin = 1e300, umax = -1e300;
+      for (int k = 0; k < 4; ++k) {
+        double fx = ix + ((k & 1) ? g.h : -g.h), fy = iy + ((k & 2) ? g.h : -g.h);
+        double wx = fx * g.dx + vg.w0x, wy = fy * g.dy + vg.w0y;
+        double uu = -(vg.e0x * wy - vg.e0y * wx) / (vg.pux * wy - vg.puy * wx);
+        umin = fmin(umin, uu);
+        umax = fmax(umax, uu);
+      }
+      int lo = (int)fmin(fmax(floor(umin), 0.0), (double)(g.nu - 1));
+      int hi = (int)fmin(fmax(floor(umax), 0.0), (double)(g.nu - 1));
+      c += (unsigned long long)(hi - lo + 1);
+    }
+  }
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, float* sino, int batch,
+                          hipStream_t s) {
+  const int waves = g.np * g.nchunk;
+  dim3 grid((waves + 3) / 4, batch), block(256);
+  if (basis == BASIS_B1) fwd_bin_kernel<BASIS_B1><<<grid, block, 0, s>>>(g, img, sino);
+  else if (basis == BASIS_B0) fwd_bin_kernel<BASIS_B0><<<grid, block, 0, s>>>(g, img, sino);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_backward(int basis, const Proj2DArgs& g, const float* sino, float* img, int batch,
+                           hipStream_t s) {
+  dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, batch), block(256);
+  if (basis == BASIS_B1) bwd_pix_kernel<BASIS_B1><<<grid, block, 0, s>>>(g, sino, img);
+  else if (basis == BASIS_B0) bwd_pix_kernel<BASIS_B0><<<grid, block, 0, s>>>(g, sino, img);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const Proj2DArgs& g, unsigned long long* out, hipStream_t s) {
+  dim3 grid((g.nx + 15) / 16, (g.ny + 15) / 16), block(256);
+  count_kernel<<<grid, block, 0, s>>>(g, out);
+  return hipGetLastError();
+}
+
+}  // namespace sgdct

@@ -1,0 +1,51 @@
+// projector.hpp -- host/device data layout of the 2-D fan-beam projectors.
+//
+// Geometry convention (utils/geo.cu:232-262 of the reference): view angle b,
+// source p = R(b)(-lso + drho, 0), detector centre R(b)(lsd - lso, 0),
+// detector step pu = R(b)(0, du); bin u spans detector coordinate [u, u+1),
+// i.e. the segment between edge points E_u and E_{u+1},
+// E_k = dtv + pu (k - nu/2).  Pixel (ix, iy) has centre
+// ((ix - (nx-1)/2) dx, (iy - (ny-1)/2) dy)  (layers/A_B1.cu:96-98).
+//
+// Edge functions.  N_k(X) = cross(E_k - p, X - p) is zero on the ray through
+// edge k and equals |beta(X)| (u(X) - k) for any point X in front of the
+// source, u(X) its continuous detector coordinate.  All footprint geometry is
+// written with N_k: the reference's corner rule for the bin range of a pixel
+// (layers/A_B1.cu:100-151) becomes two affine tests, and the edge-ray offsets
+// of layers/A_B1.cu:182-195 become ta = Ka_u N_u, tb = Kb_u N_{u+1}.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sgdct {
+
+enum Basis : int { BASIS_B0 = 0, BASIS_B1 = 1, BASIS_SF = 2, BASIS_H0 = 3, BASIS_HL = 4 };
+
+// Per-view FP64 constants.  With w = X - p = (ix dx + w0x, iy dy + w0y):
+//   N_k(ix, iy) = cross(e0 + k pu, w),   beta(ix, iy) = cross(pu, w) (< 0).
+struct ViewGeo {
+  double e0x, e0y;  // E_0 - p
+  double pux, puy;  // detector step
+  double w0x, w0y;  // (X - p) at pixel index (0, 0)
+  double px, py;    // source
+  // FP32 gradients (index units) of N_0 and beta, and the bound
+  // db = h (|d beta/d ix| + |d beta/d iy|) used by the per-pixel bin range.
+  float gx0, gy0, gx1, gy1, db, pad0, pad1, pad2;
+};
+
+// Arguments shared by every 2-D kernel.
+struct Proj2DArgs {
+  const ViewGeo* views;  // [np]
+  const void* bins;      // [np][nu] B0Bin / B1Bin (unused for SF)
+  int nx, ny, np, nu;
+  double dx, dy;
+  float h;               // corner half-extent in index units (1 for B1, 1/2 for B0/SF)
+  int nchunk;            // ceil(nu / 64)
+};
+
+constexpr int kMaxWindow = 128;   // bins of one view cached in LDS per pixel tile
+constexpr int kViewChunk = 4;     // views per LDS refill in the gather kernels
+constexpr int kTile = 16;         // pixel tile edge of the gather kernels
+
+}  // namespace sgdct

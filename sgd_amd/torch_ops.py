@@ -1,0 +1,133 @@
+"""Device-resident FP32 projectors on torch tensors, and A as an autograd op.
+
+`Projector` owns one C-ABI handle and runs A / A^T on CUDA(HIP) tensors on
+torch's current stream (sgdct_forward_dev / sgdct_backward_dev): images are
+(..., ny, nx) and sinograms (..., np, nu), FP32, contiguous.  `project` is A
+wrapped as a torch.autograd.Function whose backward is the hand-written A^T
+kernel (no autograd tracing through the footprint arithmetic).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+BASES = {"B0": "b0", "B1": "b1", "SF": "SF"}
+
+
+class Projector:
+    def __init__(self, basis, nx, ny, angles, nu, dx, dy, du, lsd, lso, drho=None):
+        if basis not in BASES:
+            raise ValueError("unknown basis %r" % (basis,))
+        if not torch.cuda.is_available():
+            raise _lib.SgdctError("sgd_amd projectors need a ROCm GPU (torch.cuda unavailable)")
+        self.basis = basis
+        self.nx, self.ny, self.nu = int(nx), int(ny), int(nu)
+        ang = np.ascontiguousarray(angles, dtype=np.float64)
+        self.np = int(ang.size)
+        self.angles = ang
+        self.geom = dict(dx=float(dx), dy=float(dy), du=float(du), lsd=float(lsd), lso=float(lso))
+        L = _lib.lib()
+        torch.cuda.init()
+        pre = BASES[basis]
+        if pre == "SF":
+            if drho is not None:
+                raise ValueError("SF_init takes no drho (layers/A_SF.cu:396)")
+            h = L.SF_init(self.nx, self.ny, self.np, self.nu, dx, dy, du, lsd, lso, _lib.dptr(ang))
+        else:
+            dr = None if drho is None else np.ascontiguousarray(drho, dtype=np.float64)
+            self._drho = dr
+            h = getattr(L, pre + "_init")(self.nx, self.ny, self.np, self.nu, dx, dy, du, lsd, lso,
+                                          _lib.dptr(ang), None, None if dr is None else _lib.dptr(dr))
+        if not h:
+            raise _lib.SgdctError("%s_init failed: %s" % (pre, _lib.last_error()))
+        self.handle = h
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().sgdct_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def img_shape(self):
+        return (self.ny, self.nx)
+
+    @property
+    def sino_shape(self):
+        return (self.np, self.nu)
+
+    def eval_count(self):
+        return _lib.check(_lib.lib().sgdct_eval_count(self.handle), "eval_count")
+
+    def _check_in(self, t, tail):
+        if not (t.is_cuda and t.dtype == torch.float32):
+            raise TypeError("expected a float32 CUDA tensor")
+        if tuple(t.shape[-2:]) != tail:
+            raise ValueError("trailing shape %s, expected %s" % (tuple(t.shape[-2:]), tail))
+        return t.contiguous()
+
+    def forward(self, img, out=None):
+        img = self._check_in(img, self.img_shape)
+        lead = img.shape[:-2]
+        batch = int(np.prod(lead)) if len(lead) else 1
+        if out is None:
+            out = torch.empty(lead + self.sino_shape, device=img.device, dtype=torch.float32)
+        stream = torch.cuda.current_stream(img.device).cuda_stream
+        _lib.check(_lib.lib().sgdct_forward_dev(self.handle, ctypes.c_void_p(img.data_ptr()),
+                                                ctypes.c_void_p(out.data_ptr()), batch,
+                                                ctypes.c_void_p(stream)), "forward")
+        return out
+
+    def backward(self, sino, out=None):
+        sino = self._check_in(sino, self.sino_shape)
+        lead = sino.shape[:-2]
+        batch = int(np.prod(lead)) if len(lead) else 1
+        if out is None:
+            out = torch.empty(lead + self.img_shape, device=sino.device, dtype=torch.float32)
+        stream = torch.cuda.current_stream(sino.device).cuda_stream
+        _lib.check(_lib.lib().sgdct_backward_dev(self.handle, ctypes.c_void_p(sino.data_ptr()),
+                                                 ctypes.c_void_p(out.data_ptr()), batch,
+                                                 ctypes.c_void_p(stream)), "backward")
+        return out
+
+    __call__ = forward
+
+
+class _ProjectFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, img, proj):
+        ctx.proj = proj
+        return proj.forward(img)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.proj.backward(grad.contiguous()), None
+
+
+class _BackProjectFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sino, proj):
+        ctx.proj = proj
+        return proj.backward(sino)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.proj.forward(grad.contiguous()), None
+
+
+def project(img, proj):
+    """A as a differentiable op; d/d img is the A^T kernel."""
+    return _ProjectFn.apply(img, proj)
+
+
+def backproject(sino, proj):
+    """A^T as a differentiable op; d/d sino is the A kernel."""
+    return _BackProjectFn.apply(sino, proj)

@@ -1,0 +1,31 @@
+"""The C-ABI library loads on a CPU-only host and exports every function that
+include/sgdct.h declares (no compute calls: there is no GPU here)."""
+import ctypes
+import os
+
+import pytest
+
+from sgd_amd import _lib
+
+
+def test_header_declares_reference_entry_points():
+    names = set(_lib.header_functions())
+    for pre in ("b0", "b1", "SF"):
+        assert "%s_forward_projection" % pre in names
+        assert "%s_backward_projection" % pre in names
+    assert {"b0_init", "b1_init", "SF_init", "sgdct_destroy", "sgdct_forward_dev",
+            "sgdct_backward_dev", "sgdct_last_error"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libsgdct.so not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in _lib.header_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_sets_signatures():
+    L = _lib.lib()
+    assert L.b1_init.restype is ctypes.c_void_p
+    assert L.sgdct_last_error.restype is ctypes.c_char_p
