@@ -676,12 +676,13 @@ static double tau_of(const ogeo *g, int p, int u, double X, double Y) {
     return uu - u;
 }
 
-static const double GL8x[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290,
-                               -0.1834346424956498, 0.1834346424956498,  0.5255324099163290,
-                               0.7966664774136267,  0.9602898564975363};
-static const double GL8w[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066661925261,
-                               0.3626837833783620, 0.3626837833783620, 0.3137066661925261,
-                               0.2223810344533745, 0.1012285362903763};
+/* 8-point Gauss-Legendre rule on [-1, 1] */
+static const double GL8x[8] = {-0.96028985649753623, -0.79666647741362674, -0.52553240991632899,
+                               -0.18343464249564980, 0.18343464249564980,  0.52553240991632899,
+                               0.79666647741362674,  0.96028985649753623};
+static const double GL8w[8] = {0.10122853629037626, 0.22238103445337447, 0.31370664587788729,
+                               0.36268378337836198, 0.36268378337836198, 0.31370664587788729,
+                               0.22238103445337447, 0.10122853629037626};
 
 /* Exact average over bin u of view p of the basis line integral for pixel
  * (ix, iy), unit coefficient.  Gauss-Legendre (8 points) on the sub-intervals
@@ -722,7 +723,10 @@ double or_strip_exact(int basis, const void *h, int p, int u, int ix, int iy) {
 
 /* Adaptive Simpson over the bin with Richardson correction (q + (q-v)/15),
  * the 1-D analogue of layers/Ref_projector.cu:156-214; returns the average
- * over the bin and writes the accumulated |q-v|/15 error estimate. */
+ * over the bin and writes the accumulated |q-v|/15 error estimate.  The
+ * first four levels always subdivide: the integrand has kinks (rays through
+ * the basis function's corners) that a coarse panel can miss entirely. */
+static int ORACLE_SIMPSON_FORCED = 0; /* depth at or below which panels may stop */
 static double simpson_rec(int basis, const ogeo *g, int p, double ax, double ay, double pux,
                           double puy, double cx, double cy, double t0, double t1, double f0,
                           double fm, double f1, double whole, double eps, int depth, double *err) {
@@ -734,7 +738,7 @@ static double simpson_rec(int basis, const ogeo *g, int p, double ax, double ay,
     double left = (tm - t0) * (f0 + 4 * fl + fm) / 6.0;
     double right = (t1 - tm) * (fm + 4 * fr + f1) / 6.0;
     double q = left + right;
-    if (depth <= 0 || fabs(q - whole) <= 15.0 * eps) {
+    if (depth <= 0 || (depth <= ORACLE_SIMPSON_FORCED && fabs(q - whole) <= 15.0 * eps)) {
         *err += fabs(q - whole) / 15.0;
         return q + (q - whole) / 15.0;
     }
@@ -742,6 +746,7 @@ static double simpson_rec(int basis, const ogeo *g, int p, double ax, double ay,
            simpson_rec(basis, g, p, ax, ay, pux, puy, cx, cy, tm, t1, fm, fr, f1, right, 0.5 * eps, depth - 1, err);
 }
 
+#define ORACLE_SIMPSON_FORCED_LEVELS 4
 double or_strip_simpson(int basis, const void *h, int p, int u, int ix, int iy, double eps, int depth,
                         double *err_out) {
     const ogeo *g = (const ogeo *)h;
@@ -755,6 +760,7 @@ double or_strip_simpson(int basis, const void *h, int p, int u, int ix, int iy, 
     double fm = ray_value(basis, px, py, ax + 0.5 * pux, ay + 0.5 * puy, cx, cy, dx, dy);
     double whole = (f0 + 4 * fm + f1) / 6.0;
     double err = 0.0;
+    ORACLE_SIMPSON_FORCED = depth - ORACLE_SIMPSON_FORCED_LEVELS;
     double v = simpson_rec(basis, g, p, ax, ay, pux, puy, cx, cy, 0.0, 1.0, f0, fm, f1, whole, eps, depth, &err);
     if (err_out) *err_out = err;
     return v;
