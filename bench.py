@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--basis", default="B1", choices=["B1", "B0", "SF"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=180)
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -94,6 +96,12 @@ def main():
     xo = torch.empty_like(x)
     stream = torch.cuda.current_stream(dev)
 
+    if args.copy_calib:
+        src = torch.ones(args.copy_calib * (1 << 18), dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         P.forward(x, out=yo)
         P.backward(y, out=xo)

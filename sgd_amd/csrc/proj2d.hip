@@ -76,22 +76,33 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float*
   const double acx = -eay * g.dx, acy = eax * g.dy, ac0 = eax * vg.w0y - eay * vg.w0x;
   const double bcx = -eby * g.dx, bcy = ebx * g.dy, bc0 = ebx * vg.w0y - eby * vg.w0x;
   // Corner rule: bin u is in a pixel's range iff N_u + R_u >= 0 and
-  // N_{u+1} - R_{u+1} < 0 (max / min of N over the pixel's corners).
-  const double ra = g.h * (fabs(acx) + fabs(acy)), rb = g.h * (fabs(bcx) + fabs(bcy));
+  // N_{u+1} - R_{u+1} < 0 (max / min of N over the pixel's corners).  The
+  // reference clamps ranges into [0, nu-1] (layers/A_B1.cu:150-151), so bin 0
+  // also serves pixels projecting before the detector and bin nu-1 pixels
+  // projecting past it; one extra bin of margin covers every pixel whose
+  // footprint can still reach the end bins.
+  const double lax = (u == 0) ? eax - vg.pux : eax, lay = (u == 0) ? eay - vg.puy : eay;
+  const double lbx = (u == g.nu - 1) ? ebx + vg.pux : ebx, lby = (u == g.nu - 1) ? eby + vg.puy : eby;
+  const double qcx = -lay * g.dx, qcy = lax * g.dy, qc0 = lax * vg.w0y - lay * vg.w0x;
+  const double rcx = -lby * g.dx, rcy = lbx * g.dy, rc0 = lbx * vg.w0y - lby * vg.w0x;
+  const double ra = g.h * (fabs(qcx) + fabs(qcy)), rb = g.h * (fabs(rcx) + fabs(rcy));
+  // ta / tb step along the walk (edges u, u+1); pixel-set tests (edges q, r)
   const double sa = rows ? acx : acy, ma = rows ? acy : acx;
   const double sb = rows ? bcx : bcy, mb = rows ? bcy : bcx;
+  const double sq = rows ? qcx : qcy, mq = rows ? qcy : qcx;
+  const double sr = rows ? rcx : rcy, mr = rows ? rcy : rcx;
   const int nmin = rows ? g.nx : g.ny, nmaj = rows ? g.ny : g.nx;
   const size_t smin = rows ? 1 : g.nx, smaj = rows ? g.nx : 1;
-  const double isa = sa != 0.0 ? 1.0 / sa : 0.0, isb = sb != 0.0 ? 1.0 / sb : 0.0;
+  const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
   const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
 
   // Lines crossed by the strip: between the extreme crossings of its two
   // boundary lines with the first and last pixel of a line.
   int m_lo = 0, m_hi = nmaj - 1;
-  if (ma != 0.0 && mb != 0.0) {
+  if (mq != 0.0 && mr != 0.0) {
     const double last = nmin - 1;
-    double c1 = (-ra - ac0) / ma, c2 = (-ra - ac0 - sa * last) / ma;
-    double c3 = (rb - bc0) / mb, c4 = (rb - bc0 - sb * last) / mb;
+    double c1 = (-ra - qc0) / mq, c2 = (-ra - qc0 - sq * last) / mq;
+    double c3 = (rb - rc0) / mr, c4 = (rb - rc0 - sr * last) / mr;
     double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
     lo = fmax(floor(lo) - 1.0, 0.0);
     hi = fmin(ceil(hi) + 1.0, (double)(nmaj - 1));
@@ -101,14 +112,14 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float*
 
   double acc = 0.0;
   for (int m = m_lo; m <= m_hi; ++m) {
-    const double r1 = -ra - ma * m - ac0;  // need sa * n >= r1
-    const double r2 = rb - mb * m - bc0;   // need sb * n <  r2
+    const double r1 = -ra - mq * m - qc0;  // need sq * n >= r1
+    const double r2 = rb - mr * m - rc0;   // need sr * n <  r2
     double lo = 0.0, hi = (double)(nmin - 1);
-    if (sa > 0.0) lo = fmax(lo, ceil(r1 * isa));
-    else if (sa < 0.0) hi = fmin(hi, floor(r1 * isa));
+    if (sq > 0.0) lo = fmax(lo, ceil(r1 * isq));
+    else if (sq < 0.0) hi = fmin(hi, floor(r1 * isq));
     else if (r1 > 0.0) continue;
-    if (sb > 0.0) hi = fmin(hi, ceil(r2 * isb) - 1.0);
-    else if (sb < 0.0) lo = fmax(lo, floor(r2 * isb) + 1.0);
+    if (sr > 0.0) hi = fmin(hi, ceil(r2 * isr) - 1.0);
+    else if (sr < 0.0) lo = fmax(lo, floor(r2 * isr) + 1.0);
     else if (r2 <= 0.0) continue;
     if (lo > hi) continue;
     const int n0 = (int)lo, n1 = (int)hi;
@@ -215,9 +226,12 @@ __global__ __launch_bounds__(256) void bwd_pix_kernel(Proj2DArgs g, const float*
         const float gx = __builtin_fmaf(kc, vg.gx1, vg.gx0), gy = __builtin_fmaf(kc, vg.gy1, vg.gy0);
         const float hw = g.h * (__builtin_fabsf(gx) + __builtin_fabsf(gy)) *
                          __builtin_amdgcn_rcpf(__builtin_fabsf(bef) - vg.db);
-        const float flo = floorf(kc - hw), fhi = floorf(kc + hw);
-        if (fhi < 0.0f || flo > (float)(g.nu - 1)) continue;
-        const int ulo = max((int)flo, 0), uhi = min((int)fhi, g.nu - 1);
+        // clamped corner range (layers/A_B1.cu:150-151): pixels projecting
+        // wholly before / past the detector still visit bin 0 / nu-1
+        const float last = (float)(g.nu - 1);
+        const float flo = fminf(fmaxf(floorf(kc - hw), 0.0f), last);
+        const float fhi = fminf(fmaxf(floorf(kc + hw), 0.0f), last);
+        const int ulo = (int)flo, uhi = (int)fhi;
         float na = (float)(n0 + (double)ulo * be);
         const int wlo = win_lo[j], wn = win_n[j];
         float accv = 0.0f;

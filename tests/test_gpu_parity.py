@@ -8,6 +8,16 @@ its FP64 build ("exact") evaluates them without rounding noise, its FP32 build
     (times 1.5), i.e. within the reference's own rounding noise
   * adjoint |<Ax,y> - <x,A^T y>| / (|Ax||y|) <= 1e-6
 Each case prints its name before touching the GPU.
+
+Hand-made geometries that contain axis-aligned views use an odd pixel count
+and a source distance like 500.3 mm.  With an even count the pixel corners on
+the central axis project exactly onto the central bin edge at 0/90/180/270
+degrees (and with "round" magnifications other corners land exactly on other
+edges); there the reference's B0 corner rule floor(u) (layers/A_B0.cu:194-195)
+is discontinuous, its footprint reaches past the corner range, and any two
+implementations -- the reference's own FP32 build included -- may keep or drop
+a bin with a nonzero value.  The real configurations (config1 here) are tested
+at their exact numbers, where such ties weigh ~4e-7.
 """
 import numpy as np
 import pytest
@@ -27,14 +37,14 @@ def rel(a, b):
 CASES = {
     # name: (nx, ny, angles, nu, dx, dy, du, lsd, lso, drho)
     "tiny": (64, 64, 2 * np.pi * np.arange(90) / 90, 96, 2.4, 2.4, 3.2, 1000.0, 500.0, None),
-    "diagonals": (48, 48, np.array([0, np.pi / 4, np.pi / 2, 3 * np.pi / 4, np.pi, 5 * np.pi / 4,
+    "diagonals": (47, 47, np.array([0, np.pi / 4, np.pi / 2, 3 * np.pi / 4, np.pi, 5 * np.pi / 4,
                                      3 * np.pi / 2, 7 * np.pi / 4, 0.3, 2.0, -0.7, 4.0 + 1e-9]),
-                  80, 3.0, 3.0, 3.5, 1000.0, 500.0, None),
+                  80, 3.0, 3.0, 3.5, 1000.0, 500.3, None),
     "nonsquare": (96, 64, 2 * np.pi * np.arange(60) / 60 + 0.01, 128, 1.5, 2.0, 2.2, 800.0, 350.0, None),
-    "offdetector": (64, 64, 2 * np.pi * np.arange(40) / 40, 24, 2.4, 2.4, 3.0, 1000.0, 500.0, None),
-    "drho": (64, 64, 2 * np.pi * np.arange(36) / 36, 96, 2.4, 2.4, 3.2, 1000.0, 500.0,
+    "offdetector": (63, 63, 2 * np.pi * np.arange(40) / 40, 24, 2.4, 2.4, 3.0, 1000.0, 500.3, None),
+    "drho": (63, 63, 2 * np.pi * np.arange(36) / 36, 96, 2.4, 2.4, 3.2, 1000.0, 500.0,
              5.0 * np.sin(np.arange(36))),
-    "tight": (64, 64, 2 * np.pi * np.arange(45) / 45, 160, 0.5, 0.5, 0.4, 60.0, 30.0, None),
+    "tight": (64, 64, 2 * np.pi * np.arange(45) / 45, 160, 0.5, 0.5, 0.4, 60.0, 30.07, None),
     "config1": (256, 256, 2 * np.pi * np.arange(360) / 360, 512, 1.0, 1.0, 1.0, 1000.0, 500.0, None),
 }
 
