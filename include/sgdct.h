@@ -54,6 +54,24 @@ sgdct_projector *SF_init(int nx, int ny, int np, int nu, double dx, double dy, d
 int SF_forward_projection(double *b, double *x, sgdct_projector *h);
 int SF_backward_projection(double *b, double *x, sgdct_projector *h);
 
+/* layers/A_H0.hh:24 / A_H0.cu:509 (h0_init), :556 (forward), :601 (backward);
+ * nl rows x nc columns of a hexagonal lattice, host image x[ic*nl + il]; the
+ * last column of odd rows is not a lattice point (ignored by A, zeroed by A^T).
+ * Fixed against the reference: bin ranges from the true hexagon vertices
+ * (A_H0.cu:82-156 uses the Hl row pitch), values divided by 3 (A_H0.cu:245
+ * sums three unit-integral rhombi; its matrix generator divides, :438). */
+sgdct_projector *h0_init(int nl, int nc, int np, int nu, double dt, double du, double lsd,
+                         double lso, const double *angles, const double *dz, const double *drho);
+int h0_forward_projection(double *b, double *x, sgdct_projector *h);
+int h0_backward_projection(double *b, double *x, sgdct_projector *h);
+
+/* layers/A_Hl.hh:24 / A_Hl.cu:521 (hl_init), :569 (forward), :614 (backward);
+ * three-direction box-spline basis on the row pitch dt/2 lattice. */
+sgdct_projector *hl_init(int nl, int nc, int np, int nu, double dt, double du, double lsd,
+                         double lso, const double *angles, const double *dz, const double *drho);
+int hl_forward_projection(double *b, double *x, sgdct_projector *h);
+int hl_backward_projection(double *b, double *x, sgdct_projector *h);
+
 /* ---- additions ------------------------------------------------------------ */
 
 /* Release a handle from any *_init (the reference leaks them). */
@@ -66,8 +84,8 @@ void sgdct_destroy(sgdct_projector *h);
 int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
 
-/* Shape query: out[0..3] = nx, ny, np, nu; returns the basis id
- * (0 B0, 1 B1, 2 SF) or a negative code. */
+/* Shape query: out[0..3] = nx, ny, np, nu (hexagonal: nc, nl, np, nu);
+ * returns the basis id (0 B0, 1 B1, 2 SF, 3 H0, 4 Hl) or a negative code. */
 int sgdct_shape(const sgdct_projector *h, int *out);
 
 /* Footprint evaluations of one A under the reference's corner rule (the

@@ -15,7 +15,7 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-BASES = {"B0": 0, "B1": 1, "SF": 2}
+BASES = {"B0": 0, "B1": 1, "SF": 2, "H0": 3, "Hl": 4}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -66,6 +66,17 @@ class Oracle:
             lib.or_s2tri.restype = ctypes.c_double
             lib.or_s2tri.argtypes = [ctypes.c_double] * 3
             lib.or_precision_bits.restype = ctypes.c_int
+            for name in ("or_hex_forward", "or_hex_backward"):
+                f = getattr(lib, name)
+                f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int]
+            lib.or_hex_footprint.restype = ctypes.c_int
+            lib.or_hex_footprint.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4 + [
+                _ip, _dp, ctypes.c_int]
+            lib.or_hex_strip_exact.restype = ctypes.c_double
+            lib.or_hex_strip_exact.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4
+            lib.or_hl_conv.restype = ctypes.c_double
+            lib.or_hl_conv.argtypes = [ctypes.c_double] * 5
             Oracle._libs[path] = lib
         self.lib = Oracle._libs[path]
         self.mode = mode
@@ -82,6 +93,9 @@ class Oracle:
 
     def box3(self, y, v1, v2, v3):
         return self.lib.or_box3(y, v1, v2, v3)
+
+    def hl_conv(self, y, h1, h2, h3, h4):
+        return self.lib.or_hl_conv(y, h1, h2, h3, h4)
 
 
 class Geometry:
@@ -152,3 +166,50 @@ class Geometry:
         err = ctypes.c_double(0.0)
         v = self.o.lib.or_strip_simpson(BASES[basis], self.h, p, u, ix, iy, eps, depth, ctypes.byref(err))
         return v, err.value
+
+
+class HexGeometry(Geometry):
+    """Hexagonal lattice of nl rows x nc columns (layers/A_H0.cu, A_Hl.cu):
+    odd rows are shifted by half a column and hold nc-1 points.  Images are
+    (nl, nc) arrays indexed [il, ic]; the last column of odd rows is unused."""
+
+    def __init__(self, oracle, nl, nc, angles, nu, dt, du, lsd, lso, drho=None):
+        super().__init__(oracle, nc, nl, angles, nu, dt * 3 ** 0.5, dt * 0.5, du, lsd, lso, drho=drho)
+        self.nl, self.nc, self.dt = int(nl), int(nc), float(dt)
+
+    def mask(self):
+        m = np.ones((self.nl, self.nc), dtype=bool)
+        m[1::2, -1] = False
+        return m
+
+    def forward(self, basis, img, views=None, pad=0):
+        img = np.ascontiguousarray(img, dtype=np.float64).reshape(self.nl, self.nc)
+        p0, p1 = views if views is not None else (0, self.np)
+        sino = np.zeros((self.np, self.nu))
+        self.o.lib.or_hex_forward(BASES[basis], self.h, self.dt, _ptr(img), _ptr(sino), p0, p1, pad)
+        return sino
+
+    def backward(self, basis, sino, views=None, pad=0):
+        sino = np.ascontiguousarray(sino, dtype=np.float64).reshape(self.np, self.nu)
+        p0, p1 = views if views is not None else (0, self.np)
+        img = np.zeros((self.nl, self.nc))
+        self.o.lib.or_hex_backward(BASES[basis], self.h, self.dt, _ptr(sino), _ptr(img), p0, p1, pad)
+        return img
+
+    def footprint(self, basis, p, ic, il, pad=0, cap=64):
+        us = np.zeros(cap, dtype=np.int32)
+        ws = np.zeros(cap)
+        n = self.o.lib.or_hex_footprint(BASES[basis], self.h, self.dt, p, ic, il, pad,
+                                        us.ctypes.data_as(_ip), _ptr(ws), cap)
+        n = min(n, cap)
+        return us[:n].copy(), ws[:n].copy()
+
+    def strip_exact(self, basis, p, u, ic, il):
+        return self.o.lib.or_hex_strip_exact(BASES[basis], self.h, self.dt, p, u, ic, il)
+
+    def centre(self, ic, il, basis):
+        dy = 1.5 * self.dt if basis == "H0" else 0.5 * self.dt
+        dx = 3 ** 0.5 * self.dt
+        cy = (il - 0.5 * (self.nl - 1)) * dy
+        cx = (ic - 0.5 * (self.nc - (2 if il & 1 else 1))) * dx
+        return cx, cy

@@ -789,4 +789,270 @@ void or_forward_exact(int basis, const void *h, const double *img, double *sino,
     }
 }
 
+/* ------------------------------------------------------------------------- */
+/* Hexagonal lattices (layers/A_H0.cu, layers/A_Hl.cu).                       */
+/* Lattice point (ic, il): il-th row, ic-th column, odd rows shifted by half  */
+/* a column and one point shorter (A_H0.cu:172-183, A_Hl.cu:171-182).         */
+/*   H0: hexagon indicator, circumradius dt, row pitch 1.5 dt, columns dt*sqrt3 */
+/*   Hl: three-direction box spline (Courant hat), row pitch dt/2             */
+/* Bin ranges use the six true hexagon vertices (the reference's H0 culling   */
+/* projects vertices around the Hl row pitch, A_H0.cu:82-156 with            */
+/* h0_init's dy = dt/2, and loses off-centre rows; fixed here).              */
+/* H0 values are divided by 3: the reference sums three unit-integral rhombus */
+/* splines (A_H0.cu:245), i.e. three times the hexagon's projection; its      */
+/* commented matrix generator divides by 3 (A_H0.cu:438).                     */
+/* ------------------------------------------------------------------------- */
+#define OR_SQRT3 1.7320508075688772935274463415
+
+/* FP64 truncated-power forward differences, spline_conv.hh:150-229 */
+static double tp2(double y) { return y > 0 ? y : 0; }
+static double tp3(double y) { return y > 0 ? y * y : 0; }
+static double tp4(double y) { return y > 0 ? y * y * y : 0; }
+static double fd2_2(double y, double h) { return (tp2(y) - tp2(y - h)) / h; }
+static double fd2_1(double y, double h, double h2) { return (fd2_2(y, h) - fd2_2(y - h2, h)) / h2; }
+static double fd3_3(double y, double h) { return (tp3(y) - tp3(y - h)) / h; }
+static double fd3_2(double y, double h, double h2) { return (fd3_3(y, h) - fd3_3(y - h2, h)) / h2; }
+static double fd3_1(double y, double h, double h2, double h3) { return (fd3_2(y, h, h2) - fd3_2(y - h3, h, h2)) / h3; }
+static double fd4_4(double y, double h) { return (tp4(y) - tp4(y - h)) / h; }
+static double fd4_3(double y, double h, double h2) { return (fd4_4(y, h) - fd4_4(y - h2, h)) / h2; }
+static double fd4_2(double y, double h, double h2, double h3) { return (fd4_3(y, h, h2) - fd4_3(y - h3, h, h2)) / h3; }
+static double fd4_1(double y, double h, double h2, double h3, double h4) { return (fd4_2(y, h, h2, h3) - fd4_2(y - h4, h, h2, h3)) / h4; }
+
+/* Hl footprint: layers/A_Hl.cu:21-46 (directions with |h| <= 1e-8 dropped) */
+static double hl_conv(double y, double h1, double h2, double h3, double h4) {
+    double hs[4];
+    int n = 0;
+    if (fabs(h1) > 1e-8) hs[n++] = h1;
+    if (fabs(h2) > 1e-8) hs[n++] = h2;
+    if (fabs(h3) > 1e-8) hs[n++] = h3;
+    if (fabs(h4) > 1e-8) hs[n++] = h4;
+    switch (n) {
+    case 2: return fd2_1(y, hs[0], hs[1]);
+    case 3: return fd3_1(y, hs[0], hs[1], hs[2]) / 2.0;
+    case 4: return fd4_1(y, hs[0], hs[1], hs[2], hs[3]) / 6.0;
+    default: return 0.0; /* the reference asserts here */
+    }
+}
+
+double or_hl_conv(double y, double h1, double h2, double h3, double h4) { return hl_conv(y, h1, h2, h3, h4); }
+
+enum { OR_H0 = 3, OR_HL = 4 };
+
+/* lattice point centre (physical) */
+static void hex_centre(const ogeo *g, int basis, int ic, int il, double dt, fpt *ox, fpt *oy) {
+    fpt fdt = (fpt)dt;
+    fpt dx = fdt * (fpt)OR_SQRT3;
+    fpt dy = (basis == OR_H0) ? fdt * (fpt)1.5f : fdt * (fpt)0.5f;
+    fpt nl2 = (fpt)0.5f * (fpt)(g->ny - 1), nc2 = (fpt)0.5f * (fpt)(g->nx - 1), nc3 = (fpt)0.5f * (fpt)(g->nx - 2);
+    *oy = ((fpt)il - nl2) * dy;
+    *ox = ((il & 1) ? ((fpt)ic - nc3) : ((fpt)ic - nc2)) * dx;
+}
+
+/* bin range from the six hexagon vertices (circumradius dt, pointy-top) */
+static void hex_range(const ogeo *g, int p, double ox, double oy, double dt, int *minu, int *maxu) {
+    static const double vx[6] = {0.0, 0.5, 0.5, 0.0, -0.5, -0.5};
+    static const double vy[6] = {-1.0, -0.5, 0.5, 1.0, 0.5, -0.5};
+    double umin = 1e300, umax = -1e300;
+    for (int k = 0; k < 6; ++k) {
+        double u = or_geo_u(g, p, ox + vx[k] * OR_SQRT3 * dt, oy + vy[k] * dt);
+        if (u < umin) umin = u;
+        if (u > umax) umax = u;
+    }
+    *minu = imin(imax(0, (int)floor(umin)), g->nu - 1);
+    *maxu = imin(imax(0, (int)floor(umax)), g->nu - 1);
+}
+
+/* H0 / Hl for one (lattice point, view): A_H0.cu:158-256, A_Hl.cu:168-266 */
+static void hex_pixel(const ogeo *g, int basis, double dt, int p, int ic, int il, int pad, emit_fn emit,
+                      void *ctx) {
+    int nu = g->nu;
+    fpt fdt = (fpt)dt;
+    fpt dx = fdt * (fpt)OR_SQRT3;
+    fpt dy = (basis == OR_H0) ? fdt * (fpt)1.5f : fdt * (fpt)0.5f;
+    fpt px = (fpt)g->src[2 * p], py = (fpt)g->src[2 * p + 1];
+    fpt pux = (fpt)g->puv[2 * p], puy = (fpt)g->puv[2 * p + 1];
+    fpt dtx = (fpt)g->dtv[2 * p], dty = (fpt)g->dtv[2 * p + 1];
+    fpt ox, oy;
+    int minu, maxu;
+    hex_centre(g, basis, ic, il, dt, &ox, &oy);
+    hex_range(g, p, (double)ox, (double)oy, dt, &minu, &maxu);
+    minu = imax(0, minu - pad);
+    maxu = imin(nu - 1, maxu + pad);
+    fpt sx = dtx + pux * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    fpt sy = dty + puy * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    for (int u = minu; u <= maxu; ++u, sx += pux, sy += puy) {
+        fpt ax = sx - pux * (fpt)0.5f, ay = sy - puy * (fpt)0.5f;
+        fpt bx = sx + pux * (fpt)0.5f, by = sy + puy * (fpt)0.5f;
+        fpt num = ((ox - px) * (sx - px) + (oy - py) * (sy - py));
+        fpt dena = ((ax - px) * (sx - px) + (ay - py) * (sy - py));
+        fpt denb = ((bx - px) * (sx - px) + (by - py) * (sy - py));
+        fpt ra = num / dena, rb = num / denb;
+        fpt ax1 = px + (ax - px) * ra, ay1 = py + (ay - py) * ra;
+        fpt bx1 = px + (bx - px) * rb, by1 = py + (by - py) * rb;
+        fpt th = (fpt)atan2((double)(sx - px), (double)(sy - py));
+        double va = dt * sin((double)th);
+        double vb = dt * sin((double)th + 2.0 * OR_PI / 3.0);
+        double vc = dt * sin((double)th + 4.0 * OR_PI / 3.0);
+        double vblur = -sqrt((double)(bx1 - ax1) * (bx1 - ax1) + (double)(by1 - ay1) * (by1 - ay1));
+        double y = sqrt((double)(ax1 - ox) * (ax1 - ox) + (double)(ay1 - oy) * (ay1 - oy));
+        if ((ay - py) * (ox - px) < (oy - py) * (ax - px)) y = -y;
+        double conv;
+        if (basis == OR_H0) {
+            conv = (double)b0_conv((fpt)y, (fpt)vblur, (fpt)va, (fpt)vb) + (double)b0_conv((fpt)y, (fpt)vblur, (fpt)vb, (fpt)vc) +
+                   (double)b0_conv((fpt)y, (fpt)vblur, (fpt)vc, (fpt)va);
+            conv /= 3.0;
+        } else {
+            conv = hl_conv(y, vblur, va, vb, vc);
+        }
+        if (conv == conv && conv > 0.0) emit(ctx, u, conv * (double)dx * (double)dy);
+    }
+}
+
+/* Hex A / A^T.  Image layout img[il*nc + ic] (nc = g->nx columns, nl = g->ny rows);
+ * the last column of odd rows is not a lattice point (A ignores it, A^T writes 0). */
+void or_hex_forward(int basis, const void *h, double dt, const double *img, double *sino, int p0, int p1, int pad) {
+    const ogeo *g = (const ogeo *)h;
+    int nc = g->nx, nl = g->ny, nu = g->nu;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = p0; p < p1; ++p) {
+        double *row = sino + (size_t)p * nu;
+        memset(row, 0, sizeof(double) * nu);
+        fwd_ctx f;
+        f.row = row;
+        for (int il = 0; il < nl; ++il)
+            for (int ic = 0; ic < nc - (il & 1); ++ic) {
+                f.c = img[(size_t)il * nc + ic];
+                hex_pixel(g, basis, dt, p, ic, il, pad, emit_fwd, &f);
+            }
+    }
+}
+
+void or_hex_backward(int basis, const void *h, double dt, const double *sino, double *img, int p0, int p1, int pad) {
+    const ogeo *g = (const ogeo *)h;
+    int nc = g->nx, nl = g->ny, nu = g->nu;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int il = 0; il < nl; ++il)
+        for (int ic = 0; ic < nc; ++ic) {
+            double acc = 0.0;
+            if (!((il & 1) && ic == nc - 1))
+                for (int p = p0; p < p1; ++p) {
+                    bwd_ctx b;
+                    b.row = sino + (size_t)p * nu;
+                    b.acc = 0.0;
+                    hex_pixel(g, basis, dt, p, ic, il, pad, emit_bwd, &b);
+                    acc += b.acc;
+                }
+            img[(size_t)il * nc + ic] = acc;
+        }
+}
+
+int or_hex_footprint(int basis, const void *h, double dt, int p, int ic, int il, int pad, int *u_out, double *w_out, int cap) {
+    const ogeo *g = (const ogeo *)h;
+    list_ctx l = {u_out, w_out, 0, cap};
+    hex_pixel(g, basis, dt, p, ic, il, pad, emit_list, &l);
+    return l.n;
+}
+
+/* exact hexagon-basis line integrals (FP64) ------------------------------- */
+/* pointy-top hexagon of circumradius dt: vertices at angles 90 + 60k deg;   */
+/* edge normals at 60k deg, apothem dt*sqrt(3)/2.                            */
+static double hex_norm(double x, double y, double dt) {
+    double apo = 0.5 * OR_SQRT3 * dt, m = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        double a = k * OR_PI / 3.0;
+        double v = (x * cos(a) + y * sin(a)) / apo;
+        if (v > m) m = v;
+    }
+    return m;
+}
+
+/* chord of the line P + s e through the hexagon centred at (cx, cy) */
+static double hex_chord(double Px, double Py, double ex, double ey, double cx, double cy, double dt) {
+    double apo = 0.5 * OR_SQRT3 * dt, lo = -1e300, hi = 1e300;
+    for (int k = 0; k < 6; ++k) {
+        double a = k * OR_PI / 3.0, nx = cos(a), ny = sin(a);
+        double d0 = (Px - cx) * nx + (Py - cy) * ny, de = ex * nx + ey * ny;
+        if (fabs(de) < 1e-300) {
+            if (d0 > apo) return 0.0;
+            continue;
+        }
+        double t = (apo - d0) / de;
+        if (de > 0) { if (t < hi) hi = t; } else { if (t > lo) lo = t; }
+    }
+    return hi > lo ? hi - lo : 0.0;
+}
+
+/* line integral of the Courant hat max(0, 1 - |x|_hex): piecewise linear
+ * along the line; split at crossings of the six spokes and integrate each
+ * piece with the two-point trapezoid (exact for linear pieces). */
+static double hex_hat_line(double Px, double Py, double ex, double ey, double cx, double cy, double dt) {
+    double apo = 0.5 * OR_SQRT3 * dt, lo = -1e300, hi = 1e300;
+    for (int k = 0; k < 6; ++k) {
+        double a = k * OR_PI / 3.0, nx = cos(a), ny = sin(a);
+        double d0 = (Px - cx) * nx + (Py - cy) * ny, de = ex * nx + ey * ny;
+        if (fabs(de) < 1e-300) { if (d0 > apo) return 0.0; continue; }
+        double t = (apo - d0) / de;
+        if (de > 0) { if (t < hi) hi = t; } else { if (t > lo) lo = t; }
+    }
+    if (!(hi > lo)) return 0.0;
+    double pts[16];
+    int n = 0;
+    pts[n++] = lo;
+    for (int k = 0; k < 6; ++k) { /* spokes: rays from centre at 90 + 60k deg */
+        double a = OR_PI / 2 + k * OR_PI / 3.0, sx = cos(a), sy = sin(a);
+        double den = ex * sy - ey * sx;
+        if (fabs(den) < 1e-300) continue;
+        double t = ((cx - Px) * sy - (cy - Py) * sx) / den;
+        if (t > lo && t < hi) pts[n++] = t;
+    }
+    pts[n++] = hi;
+    for (int i = 1; i < n; ++i)
+        for (int j = i; j > 0 && pts[j] < pts[j - 1]; --j) { double t = pts[j]; pts[j] = pts[j - 1]; pts[j - 1] = t; }
+    double sum = 0.0;
+    for (int i = 0; i + 1 < n; ++i) {
+        double f0 = 1.0 - hex_norm(Px + pts[i] * ex - cx, Py + pts[i] * ey - cy, dt);
+        double f1 = 1.0 - hex_norm(Px + pts[i + 1] * ex - cx, Py + pts[i + 1] * ey - cy, dt);
+        sum += 0.5 * (pts[i + 1] - pts[i]) * ((f0 > 0 ? f0 : 0) + (f1 > 0 ? f1 : 0));
+    }
+    return sum;
+}
+
+/* exact average over bin u of the line integral of the lattice point's basis */
+double or_hex_strip_exact(int basis, const void *h, double dt, int p, int u, int ic, int il) {
+    const ogeo *g = (const ogeo *)h;
+    double dy = (basis == OR_H0) ? 1.5 * dt : 0.5 * dt, dx = OR_SQRT3 * dt;
+    double cy = (il - 0.5 * (g->ny - 1)) * dy;
+    double cx = ((il & 1) ? (ic - 0.5 * (g->nx - 2)) : (ic - 0.5 * (g->nx - 1))) * dx;
+    double px = g->src[2 * p], py = g->src[2 * p + 1];
+    double pux = g->puv[2 * p], puy = g->puv[2 * p + 1];
+    double ax = g->dtv[2 * p] + pux * (u - 0.5 * g->nu), ay = g->dtv[2 * p + 1] + puy * (u - 0.5 * g->nu);
+    double cuts[16];
+    int nc = 0;
+    cuts[nc++] = 0.0;
+    for (int k = -1; k < 6; ++k) {
+        double X = cx, Y = cy;
+        if (k >= 0) { double a = OR_PI / 2 + k * OR_PI / 3.0; X += dt * cos(a); Y += dt * sin(a); }
+        double t = or_geo_u(g, p, X, Y) - u;
+        if (t > 0 && t < 1) cuts[nc++] = t;
+    }
+    cuts[nc++] = 1.0;
+    for (int i = 1; i < nc; ++i)
+        for (int j = i; j > 0 && cuts[j] < cuts[j - 1]; --j) { double t = cuts[j]; cuts[j] = cuts[j - 1]; cuts[j - 1] = t; }
+    double total = 0.0;
+    for (int k = 0; k + 1 < nc; ++k) {
+        double t0 = cuts[k], t1 = cuts[k + 1];
+        if (t1 <= t0) continue;
+        double half = 0.5 * (t1 - t0), mid = 0.5 * (t1 + t0);
+        for (int q = 0; q < 8; ++q) {
+            double t = mid + half * GL8x[q];
+            double qx = ax + pux * t - px, qy = ay + puy * t - py;
+            double nn = sqrt(qx * qx + qy * qy);
+            double v = (basis == OR_H0) ? hex_chord(px, py, qx / nn, qy / nn, cx, cy, dt)
+                                        : hex_hat_line(px, py, qx / nn, qy / nn, cx, cy, dt);
+            total += half * GL8w[q] * v;
+        }
+    }
+    return total;
+}
+
 int or_precision_bits(void) { return (int)(sizeof(fpt) * 8); }

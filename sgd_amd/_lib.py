@@ -46,7 +46,11 @@ def lib():
         f.restype = _h
     L.SF_init.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [_dp]
     L.SF_init.restype = _h
-    for pre in ("b0", "b1", "SF"):
+    for name in ("h0_init", "hl_init"):
+        f = getattr(L, name)
+        f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 4 + [_dp, _dp, _dp]
+        f.restype = _h
+    for pre in ("b0", "b1", "SF", "h0", "hl"):
         for kind in ("forward", "backward"):
             f = getattr(L, "%s_%s_projection" % (pre, kind))
             f.argtypes = [_dp, _dp, _h]

@@ -23,6 +23,13 @@ hipError_t launch_backward(int basis, const Proj2DArgs& g, const float* sino, fl
 hipError_t launch_count(const Proj2DArgs& g, unsigned long long* out, hipStream_t s);
 hipError_t launch_sf_forward(const Proj2DArgs& g, const float* img, float* sino, int batch, hipStream_t s);
 hipError_t launch_sf_backward(const Proj2DArgs& g, const float* sino, float* img, int batch, hipStream_t s);
+void hex_bin_constants(int basis, const ViewGeo& v, int nu, double dt, void* out);
+size_t hex_bin_size(int basis);
+hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const float* img, float* sino, int batch,
+                              hipStream_t s);
+hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const float* sino, float* img, int batch,
+                               hipStream_t s);
+hipError_t launch_hex_count(const Proj2DArgs& g, double dt, unsigned long long* out, hipStream_t s);
 }  // namespace sgdct
 
 using namespace sgdct;
@@ -66,6 +73,7 @@ struct sgdct_projector {
   int basis = 0;
   int nx = 0, ny = 0, np = 0, nu = 0;
   double dx = 0, dy = 0, du = 0, lsd = 0, lso = 0;
+  double dt = 0;  // hexagonal lattices: circumradius / lattice constant
   float h = 0.5f;
   int device = 0;
   ViewGeo* d_views = nullptr;
@@ -141,8 +149,10 @@ void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, vo
   }
 }
 
+bool is_hex(int basis) { return basis == BASIS_H0 || basis == BASIS_HL; }
+
 sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, double dy, double du,
-                        double lsd, double lso, const double* angles, const double* drho) {
+                        double lsd, double lso, const double* angles, const double* drho, double dt = 0.0) {
   if (nx <= 0 || ny <= 0 || np <= 0 || nu <= 0 || !(dx > 0) || !(dy > 0) || !(du > 0) ||
       !(lsd > 0) || !(lso > 0) || !angles) {
     fail(SGDCT_EINVAL, "invalid geometry arguments");
@@ -157,6 +167,7 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   h->nx = nx; h->ny = ny; h->np = np; h->nu = nu;
   h->dx = dx; h->dy = dy; h->du = du; h->lsd = lsd; h->lso = lso;
   h->h = (basis == BASIS_B1) ? 1.0f : 0.5f;
+  h->dt = dt;
 
   std::vector<ViewGeo> views(np);
   const double cxo = 0.5 * (nx - 1) * dx, cyo = 0.5 * (ny - 1) * dy;
@@ -181,9 +192,13 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   size_t rec = 0;
   if (basis == BASIS_B1) rec = sizeof(B1Bin);
   else if (basis == BASIS_B0) rec = sizeof(B0Bin);
+  else if (is_hex(basis)) rec = hex_bin_size(basis);
   std::vector<unsigned char> bins(rec * (size_t)np * nu);
-  if (rec)
-    for (int p = 0; p < np; ++p) bin_constants(basis, views[p], nu, dx, dy, bins.data() + rec * (size_t)p * nu);
+  for (int p = 0; p < np && rec; ++p) {
+    unsigned char* dst = bins.data() + rec * (size_t)p * nu;
+    if (is_hex(basis)) hex_bin_constants(basis, views[p], nu, dt, dst);
+    else bin_constants(basis, views[p], nu, dx, dy, dst);
+  }
 
   hipError_t e = hipGetDevice(&h->device);
   if (e == hipSuccess) e = hipMalloc(&h->d_views, sizeof(ViewGeo) * np);
@@ -218,14 +233,16 @@ int ensure_staging(sgdct_projector* h) {
 
 int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
   hipError_t e;
-  if (h->basis == BASIS_SF) e = launch_sf_forward(h->args, img, sino, batch, s);
+  if (is_hex(h->basis)) e = launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, s);
+  else if (h->basis == BASIS_SF) e = launch_sf_forward(h->args, img, sino, batch, s);
   else e = launch_forward(h->basis, h->args, img, sino, batch, s);
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "forward launch");
 }
 
 int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
   hipError_t e;
-  if (h->basis == BASIS_SF) e = launch_sf_backward(h->args, sino, img, batch, s);
+  if (is_hex(h->basis)) e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
+  else if (h->basis == BASIS_SF) e = launch_sf_backward(h->args, sino, img, batch, s);
   else e = launch_backward(h->basis, h->args, sino, img, batch, s);
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "backward launch");
 }
@@ -235,7 +252,8 @@ long long eval_count(sgdct_projector* h) {
   hipError_t e = hipSuccess;
   if (!h->d_count) e = hipMalloc(&h->d_count, sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMemset(h->d_count, 0, sizeof(unsigned long long));
-  if (e == hipSuccess) e = launch_count(h->args, h->d_count, nullptr);
+  if (e == hipSuccess)
+    e = is_hex(h->basis) ? launch_hex_count(h->args, h->dt, h->d_count, nullptr) : launch_count(h->args, h->d_count, nullptr);
   unsigned long long c = 0;
   if (e == hipSuccess) e = hipMemcpy(&c, h->d_count, sizeof(c), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(e, "evaluation count");
@@ -321,6 +339,32 @@ sgdct_projector* SF_init(int nx, int ny, int np, int nu, double dx, double dy, d
 }
 int SF_forward_projection(double* b, double* x, sgdct_projector* h) { return host_forward(b, x, h); }
 int SF_backward_projection(double* b, double* x, sgdct_projector* h) { return host_backward(b, x, h); }
+
+// layers/A_H0.hh:24 / A_H0.cu:509-643 and A_Hl.hh:24 / A_Hl.cu:521-657:
+// nl rows x nc columns, lattice constant dt; host image x[ic*nl + il].
+sgdct_projector* h0_init(int nl, int nc, int np, int nu, double dt, double du, double lsd, double lso,
+                         const double* angles, const double* dz, const double* drho) {
+  (void)dz;
+  if (!(dt > 0)) {
+    fail(SGDCT_EINVAL, "dt must be positive");
+    return nullptr;
+  }
+  return create(BASIS_H0, nc, nl, np, nu, dt * std::sqrt(3.0), 1.5 * dt, du, lsd, lso, angles, drho, dt);
+}
+int h0_forward_projection(double* b, double* x, sgdct_projector* h) { return host_forward(b, x, h); }
+int h0_backward_projection(double* b, double* x, sgdct_projector* h) { return host_backward(b, x, h); }
+
+sgdct_projector* hl_init(int nl, int nc, int np, int nu, double dt, double du, double lsd, double lso,
+                         const double* angles, const double* dz, const double* drho) {
+  (void)dz;
+  if (!(dt > 0)) {
+    fail(SGDCT_EINVAL, "dt must be positive");
+    return nullptr;
+  }
+  return create(BASIS_HL, nc, nl, np, nu, dt * std::sqrt(3.0), 0.5 * dt, du, lsd, lso, angles, drho, dt);
+}
+int hl_forward_projection(double* b, double* x, sgdct_projector* h) { return host_forward(b, x, h); }
+int hl_backward_projection(double* b, double* x, sgdct_projector* h) { return host_backward(b, x, h); }
 
 void sgdct_destroy(sgdct_projector* h) { delete h; }
 

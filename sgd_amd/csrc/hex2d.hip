@@ -1,0 +1,351 @@
+// hex2d.hip -- hexagonal-lattice projectors (H0, Hl), gfx950.
+//
+// Lattice point (ic, il): row il at y = (il - (nl-1)/2) dyh, column ic at
+// x = (ic + (il odd ? 1/2 : 0) - (nc-1)/2) dxh, dxh = sqrt(3) dt; odd rows hold
+// nc-1 points (layers/A_H0.cu:172-183, A_Hl.cu:171-182).
+//   H0: hexagon indicator (circumradius dt, dyh = 1.5 dt) = three rhombi; the
+//       value is the bin average of the sum of the three rhombus projections
+//       (box splines of two of va, vb, vc each, A_H0.cu:229-245), divided by 3
+//       so that it is the hexagon's projection (the reference omits the 1/3).
+//   Hl: three-direction box spline (dyh = dt/2), bin average of
+//       box_|va| * box_|vb| * box_|vc| (A_Hl.cu:226-236).
+// va, vb, vc = dt sin(theta + 2 pi k / 3), theta the bin's central-ray angle
+// atan2(dx, dy) (A_H0.cu:229-232).  Bin ranges come from the six true hexagon
+// vertices: the reference's H0 culling uses the Hl row pitch and drops
+// off-centre rows (A_H0.cu:82-156 with h0_init's dy = dt/2).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "footprint.hpp"
+#include "projector.hpp"
+
+namespace sgdct {
+
+// Rhombus (va, vb) of H0: trapezoid box_p * box_q centred at c = (va + vb)/2.
+struct H0Bin {
+  float ka, kb;
+  float c[3], hpq[3], hmq[3], ip[3], i2pq[3];
+  float pad[3];
+};
+
+// Hl: widths p >= q >= r (p = q + r) of box_p * box_q * box_r.
+struct HlBin {
+  float ka, kb, p, q, r, inv6pq, inv_r, pad;
+};
+
+// Antiderivative of box_p * box_q * box_r (p = q + r), odd in t:
+//   S(x) = 1/2 - [(p-x)_+^3 - (q-x)_+^3 - (r-x)_+^3] / (6pqr),   x = |t|,
+// written without the cancellations of the difference of cubes:
+//   x <= q :  1/2 - (A^2 + AB + B^2)/(6pq) + C^2 (C/r)/(6pq)
+//   x >  q :  1/2 - A^2 (A/r)/(6pq)
+// with A = (p-x)_+, B = (q-x)_+, C = (r-x)_+  (A - B = r when x <= q).
+__device__ __forceinline__ float s_hl(float t, const HlBin& r) {
+  const float x = __builtin_fabsf(t);
+  const float A = __builtin_fmaxf(r.p - x, 0.0f);
+  const float B = __builtin_fmaxf(r.q - x, 0.0f);
+  const float C = __builtin_fmaxf(r.r - x, 0.0f);
+  const float a2 = A * A;
+  const float v1 = (__builtin_fmaf(A, B, a2) + B * B) - C * C * (C * r.inv_r);
+  const float v2 = a2 * (A * r.inv_r);
+  const float v = ((x <= r.q) ? v1 : v2) * r.inv6pq;
+  return copysign_f(0.5f - v, t);
+}
+
+template <int HB> struct HexTraits;
+
+template <> struct HexTraits<BASIS_H0> {
+  using Bin = H0Bin;
+  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float sa = s_b0(ta - r.c[k], r.hpq[k], r.hmq[k], r.ip[k], r.i2pq[k]);
+      const float sb = s_b0(tb - r.c[k], r.hpq[k], r.hmq[k], r.ip[k], r.i2pq[k]);
+      s += sb - sa;
+    }
+    const float cv = s * (1.0f / 3.0f) * __builtin_amdgcn_rcpf(tb - ta);
+    return cv > 0.0f ? cv : 0.0f;  // conv > 0 (layers/A_H0.cu:248)
+  }
+};
+
+template <> struct HexTraits<BASIS_HL> {
+  using Bin = HlBin;
+  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
+    const float cv = (s_hl(tb, r) - s_hl(ta, r)) * __builtin_amdgcn_rcpf(tb - ta);
+    return cv > 0.0f ? cv : 0.0f;  // conv > 0 (layers/A_Hl.cu:238)
+  }
+};
+
+// max over the six vertices of a pointy-top hexagon of circumradius dt of
+// (vertex offset) . (gx, gy)
+__device__ __forceinline__ double hex_reach(double gx, double gy, double dt) {
+  return dt * fmax(fabs(gy), 0.8660254037844386 * fabs(gx) + 0.5 * fabs(gy));
+}
+
+struct HexArgs {
+  Proj2DArgs g;  // nx = nc, ny = nl, dx = dxh, dy = dyh
+  double dt;
+};
+
+// A: bin-driven gather over lattice rows.
+template <int HB>
+__global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* __restrict__ img,
+                                                      float* __restrict__ sino) {
+  using T = HexTraits<HB>;
+  const Proj2DArgs& g = ha.g;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int v = wave / g.nchunk;
+  if (v >= g.np) return;
+  const int u = (wave - v * g.nchunk) * 64 + lane;
+  if (u >= g.nu) return;
+  img += (size_t)blockIdx.y * g.nx * g.ny;
+  sino += (size_t)blockIdx.y * g.np * g.nu;
+  const ViewGeo vg = g.views[v];
+  const typename T::Bin rec = reinterpret_cast<const typename T::Bin*>(g.bins)[(size_t)v * g.nu + u];
+
+  const double eax = vg.e0x + u * vg.pux, eay = vg.e0y + u * vg.puy;
+  const double ebx = eax + vg.pux, eby = eay + vg.puy;
+  // N(ic', il) = s * ic' + m * il + c, ic' = ic + (il odd) / 2
+  const double sa = -eay * g.dx, ma = eax * g.dy, ca = eax * vg.w0y - eay * vg.w0x;
+  const double sb = -eby * g.dx, mb = ebx * g.dy, cb = ebx * vg.w0y - eby * vg.w0x;
+  // pixel-set edges (clamped end bins see one more bin), as in fwd_bin_kernel
+  const double lax = (u == 0) ? eax - vg.pux : eax, lay = (u == 0) ? eay - vg.puy : eay;
+  const double lbx = (u == g.nu - 1) ? ebx + vg.pux : ebx, lby = (u == g.nu - 1) ? eby + vg.puy : eby;
+  const double sq = -lay * g.dx, mq = lax * g.dy, cq = lax * vg.w0y - lay * vg.w0x;
+  const double sr = -lby * g.dx, mr = lbx * g.dy, cr = lbx * vg.w0y - lby * vg.w0x;
+  const double rq = hex_reach(-lay, lax, ha.dt), rr = hex_reach(-lby, lbx, ha.dt);
+  const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
+  const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
+
+  double acc = 0.0;
+  for (int il = 0; il < g.ny; ++il) {
+    const double sh = (il & 1) ? 0.5 : 0.0;
+    const double r1 = -rq - mq * il - cq - sq * sh;  // need sq * ic >= r1
+    const double r2 = rr - mr * il - cr - sr * sh;   // need sr * ic <  r2
+    double lo = 0.0, hi = (double)(g.nx - 1 - (il & 1));
+    if (sq > 0.0) lo = fmax(lo, ceil(r1 * isq));
+    else if (sq < 0.0) hi = fmin(hi, floor(r1 * isq));
+    else if (r1 > 0.0) continue;
+    if (sr > 0.0) hi = fmin(hi, ceil(r2 * isr) - 1.0);
+    else if (sr < 0.0) lo = fmax(lo, floor(r2 * isr) + 1.0);
+    else if (r2 <= 0.0) continue;
+    if (lo > hi) continue;
+    const int n0 = (int)lo, n1 = (int)hi;
+    float ta = (float)((double)rec.ka * (sa * (n0 + sh) + ma * il + ca));
+    float tb = (float)((double)rec.kb * (sb * (n0 + sh) + mb * il + cb));
+    const float* p = img + (size_t)il * g.nx + n0;
+    float accl = 0.0f;
+    for (int n = n0; n <= n1; ++n) {
+      accl = __builtin_fmaf(T::conv(rec, ta, tb), *p, accl);
+      ++p;
+      ta += dta;
+      tb += dtb;
+    }
+    acc += (double)accl;
+  }
+  sino[(size_t)v * g.nu + u] = (float)(acc * g.dx * g.dy);
+}
+
+// A^T: lattice-point-driven gather, 16 x 16 point tiles, LDS bin windows.
+template <int HB>
+__global__ __launch_bounds__(256) void hex_bwd_kernel(HexArgs ha, const float* __restrict__ sino,
+                                                      float* __restrict__ img) {
+  using T = HexTraits<HB>;
+  using Bin = typename T::Bin;
+  constexpr int BIN4 = sizeof(Bin) / 16;
+  static_assert(sizeof(Bin) % 16 == 0, "bin records are whole float4s");
+  __shared__ float4 lds[kViewChunk][kMaxWindow][BIN4];
+  __shared__ float ldsy[kViewChunk][kMaxWindow];
+  __shared__ int win_lo[kViewChunk], win_n[kViewChunk];
+  const Proj2DArgs& g = ha.g;
+  const int ic0 = blockIdx.x * kTile, il0 = blockIdx.y * kTile;
+  const int ic = ic0 + (threadIdx.x & (kTile - 1)), il = il0 + threadIdx.x / kTile;
+  sino += (size_t)blockIdx.z * g.np * g.nu;
+  img += (size_t)blockIdx.z * g.nx * g.ny;
+  const bool inside = ic < g.nx && il < g.ny;
+  const bool valid = inside && !((il & 1) && ic == g.nx - 1);
+  const Bin* bins = reinterpret_cast<const Bin*>(g.bins);
+  const double sh = (il & 1) ? 0.5 : 0.0;
+
+  double acc = 0.0;
+  for (int v0 = 0; v0 < g.np; v0 += kViewChunk) {
+    const int nv = min(kViewChunk, g.np - v0);
+    if ((int)threadIdx.x < nv) {
+      // window: the tile's bounding box widened by one circumradius
+      const ViewGeo& vg = g.views[v0 + threadIdx.x];
+      double umin = 1e300, umax = -1e300;
+      for (int c = 0; c < 4; ++c) {
+        double fx = (c & 1) ? (ic0 + kTile + 0.5) * g.dx + ha.dt : (ic0 - 0.5) * g.dx - ha.dt;
+        double fy = (c & 2) ? (il0 + kTile) * g.dy + ha.dt : (il0 - 1) * g.dy - ha.dt;
+        double wx = fx + vg.w0x, wy = fy + vg.w0y;
+        double uu = -(vg.e0x * wy - vg.e0y * wx) / (vg.pux * wy - vg.puy * wx);
+        umin = fmin(umin, uu);
+        umax = fmax(umax, uu);
+      }
+      int lo = (int)fmax(floor(umin) - 1.0, 0.0), hi = (int)fmin(floor(umax) + 1.0, (double)(g.nu - 1));
+      win_lo[threadIdx.x] = lo;
+      win_n[threadIdx.x] = hi >= lo ? min(hi - lo + 1, kMaxWindow) : 0;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nv * kMaxWindow; idx += blockDim.x) {
+      const int j = idx / kMaxWindow, i = idx - j * kMaxWindow;
+      if (i < win_n[j]) {
+        const size_t gi = (size_t)(v0 + j) * g.nu + win_lo[j] + i;
+        const float4* src = reinterpret_cast<const float4*>(bins + gi);
+#pragma unroll
+        for (int k = 0; k < BIN4; ++k) lds[j][i][k] = src[k];
+        ldsy[j][i] = sino[gi];
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      for (int j = 0; j < nv; ++j) {
+        const int v = v0 + j;
+        const ViewGeo& vg = g.views[v];
+        const double wx = (ic + sh) * g.dx + vg.w0x, wy = il * g.dy + vg.w0y;
+        const double n0 = vg.e0x * wy - vg.e0y * wx;
+        const double be = vg.pux * wy - vg.puy * wx;
+        const double kc = -n0 / be;
+        // k*(vertex) - kc = -(d . G) / (beta + d . grad beta), G = grad N_0 + kc grad beta
+        const double gx = -(vg.e0y + kc * vg.puy), gy = vg.e0x + kc * vg.pux;
+        const double hw = hex_reach(gx, gy, ha.dt) / (fabs(be) - hex_reach(-vg.puy, vg.pux, ha.dt));
+        const double last = (double)(g.nu - 1);
+        const int ulo = (int)fmin(fmax(floor(kc - hw), 0.0), last);
+        const int uhi = (int)fmin(fmax(floor(kc + hw), 0.0), last);
+        float na = (float)(n0 + (double)ulo * be);
+        const float bef = (float)be;
+        const int wlo = win_lo[j], wn = win_n[j];
+        float accv = 0.0f;
+        for (int u = ulo; u <= uhi; ++u) {
+          const int wi = u - wlo;
+          Bin rec;
+          float yv;
+          if (wi >= 0 && wi < wn) {
+            float4* d = reinterpret_cast<float4*>(&rec);
+#pragma unroll
+            for (int k = 0; k < BIN4; ++k) d[k] = lds[j][wi][k];
+            yv = ldsy[j][wi];
+          } else {
+            rec = bins[(size_t)v * g.nu + u];
+            yv = sino[(size_t)v * g.nu + u];
+          }
+          const float nb = na + bef;
+          accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), yv, accv);
+          na = nb;
+        }
+        acc += (double)accv;
+      }
+    }
+    __syncthreads();
+  }
+  if (inside) img[(size_t)il * g.nx + ic] = valid ? (float)(acc * g.dx * g.dy) : 0.0f;
+}
+
+// Evaluation count (clamped six-vertex ranges).
+__global__ __launch_bounds__(256) void hex_count_kernel(HexArgs ha, unsigned long long* out) {
+  __shared__ unsigned long long part[256];
+  const Proj2DArgs& g = ha.g;
+  const int ic = blockIdx.x * 16 + (threadIdx.x & 15), il = blockIdx.y * 16 + (threadIdx.x >> 4);
+  unsigned long long c = 0;
+  if (ic < g.nx && il < g.ny && !((il & 1) && ic == g.nx - 1)) {
+    const double cx = (ic + ((il & 1) ? 0.5 : 0.0)) * g.dx, cy = il * g.dy;
+    for (int v = 0; v < g.np; ++v) {
+      const ViewGeo& vg = g.views[v];
+      double umin = 1e300, umax = -1e300;
+      for (int k = 0; k < 6; ++k) {
+        const double a = 1.5707963267948966 + k * 1.0471975511965976;
+        const double wx = cx + ha.dt * cos(a) + vg.w0x, wy = cy + ha.dt * sin(a) + vg.w0y;
+        const double uu = -(vg.e0x * wy - vg.e0y * wx) / (vg.pux * wy - vg.puy * wx);
+        umin = fmin(umin, uu);
+        umax = fmax(umax, uu);
+      }
+      const int lo = (int)fmin(fmax(floor(umin), 0.0), (double)(g.nu - 1));
+      const int hi = (int)fmin(fmax(floor(umax), 0.0), (double)(g.nu - 1));
+      c += (unsigned long long)(hi - lo + 1);
+    }
+  }
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+// Per-(view, bin) constants (host side, FP64).
+void hex_bin_constants(int basis, const ViewGeo& v, int nu, double dt, void* out) {
+  for (int u = 0; u < nu; ++u) {
+    const double ax = v.e0x + u * v.pux, ay = v.e0y + u * v.puy;
+    const double bx = ax + v.pux, by = ay + v.puy;
+    const double cx = ax + 0.5 * v.pux, cy = ay + 0.5 * v.puy;
+    const double len = std::sqrt(cx * cx + cy * cy);
+    const double ka = -len / (ax * cx + ay * cy), kb = -len / (bx * cx + by * cy);
+    // central-ray angle, rounded to FP32 like atan2f (layers/A_H0.cu:229)
+    const double th = (double)(float)std::atan2(cx, cy);
+    double vk[3];
+    for (int k = 0; k < 3; ++k) vk[k] = dt * std::sin(th + k * 2.0 * 3.14159265358979323846 / 3.0);
+    if (basis == BASIS_H0) {
+      H0Bin r{};
+      r.ka = (float)ka;
+      r.kb = (float)kb;
+      const int pr[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+      for (int k = 0; k < 3; ++k) {
+        const double a = vk[pr[k][0]], b = vk[pr[k][1]];
+        const double p = std::max(std::fabs(a), std::fabs(b));
+        double q = std::min(std::fabs(a), std::fabs(b));
+        if (!(q > 1e-7)) q = 0.0;
+        r.c[k] = (float)(0.5 * (a + b));
+        r.hpq[k] = (float)(0.5 * (p + q));
+        r.hmq[k] = (float)(0.5 * (p - q));
+        r.ip[k] = (float)(1.0 / p);
+        r.i2pq[k] = q > 0.0 ? (float)(1.0 / (2.0 * p * q)) : 0.0f;
+      }
+      static_cast<H0Bin*>(out)[u] = r;
+    } else {
+      HlBin r{};
+      r.ka = (float)ka;
+      r.kb = (float)kb;
+      double w[3] = {std::fabs(vk[0]), std::fabs(vk[1]), std::fabs(vk[2])};
+      std::sort(w, w + 3);
+      const double p = w[2], q = w[1], rr = w[0];
+      r.p = (float)p;
+      r.q = (float)q;
+      r.r = (float)rr;
+      r.inv6pq = (float)(1.0 / (6.0 * p * q));
+      r.inv_r = rr > 1e-30 ? (float)(1.0 / rr) : 0.0f;
+      static_cast<HlBin*>(out)[u] = r;
+    }
+  }
+}
+
+size_t hex_bin_size(int basis) { return basis == BASIS_H0 ? sizeof(H0Bin) : sizeof(HlBin); }
+
+hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const float* img, float* sino, int batch,
+                              hipStream_t s) {
+  HexArgs ha{g, dt};
+  dim3 grid((g.np * g.nchunk + 3) / 4, batch), block(256);
+  if (basis == BASIS_H0) hex_fwd_kernel<BASIS_H0><<<grid, block, 0, s>>>(ha, img, sino);
+  else hex_fwd_kernel<BASIS_HL><<<grid, block, 0, s>>>(ha, img, sino);
+  return hipGetLastError();
+}
+
+hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const float* sino, float* img, int batch,
+                               hipStream_t s) {
+  HexArgs ha{g, dt};
+  dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, batch), block(256);
+  if (basis == BASIS_H0) hex_bwd_kernel<BASIS_H0><<<grid, block, 0, s>>>(ha, sino, img);
+  else hex_bwd_kernel<BASIS_HL><<<grid, block, 0, s>>>(ha, sino, img);
+  return hipGetLastError();
+}
+
+hipError_t launch_hex_count(const Proj2DArgs& g, double dt, unsigned long long* out, hipStream_t s) {
+  HexArgs ha{g, dt};
+  hex_count_kernel<<<dim3((g.nx + 15) / 16, (g.ny + 15) / 16), dim3(256), 0, s>>>(ha, out);
+  return hipGetLastError();
+}
+
+}  // namespace sgdct

@@ -165,3 +165,34 @@ class A_SF_operator(_ProjectorOperator):
             raise ValueError("angles must have nnp=%d entries" % nnp)
         self._keep = [ang]
         self._setup(_lib.lib().SF_init(nx, ny, nnp, nu, dx, dy, du, lsd, lso, ang_p))
+
+
+class _HexOperator(_ProjectorOperator):
+    def __init__(self, nx, ny, nnp, nu, dt, du, lsd, lso, angles, dz=None, drho=None, dtype=np.float64):
+        # operators.py:221-284: (nx, ny) are passed to h0_init/hl_init as (nl, nc)
+        super().__init__(dtype=dtype, shape=(nnp * nu, nx * ny))
+        self.vol_shape = (nx, ny)
+        self.proj_shape = (nnp, nu)
+        self.lsd = lsd
+        self.lso = lso
+        self.dimension = dt
+        self.blur = du
+        self.angles = angles
+        if dz is not None and drho is None:
+            raise AssertionError("Error: drho should be provided if dz is provided.")
+        ang, ang_p = _opt(angles)
+        if ang.size != nnp:
+            raise ValueError("angles must have nnp=%d entries" % nnp)
+        self._keep = [ang, _opt(dz)[0], _opt(drho)[0]]
+        init = getattr(_lib.lib(), "%s_init" % self._prefix)
+        self._setup(init(nx, ny, nnp, nu, dt, du, lsd, lso, ang_p, _opt(dz)[1], _opt(drho)[1]))
+
+
+class A_H0_operator(_HexOperator):
+    """Hexagon-indicator basis (operators.py:221-284 -> layers/A_H0.cu)."""
+    _prefix = "h0"
+
+
+class A_Hl_operator(_HexOperator):
+    """Hexagonal three-direction box-spline basis (operators.py:286-349 -> layers/A_Hl.cu)."""
+    _prefix = "hl"

@@ -13,11 +13,13 @@ import torch
 
 from . import _lib
 
-BASES = {"B0": "b0", "B1": "b1", "SF": "SF"}
+BASES = {"B0": "b0", "B1": "b1", "SF": "SF", "H0": "h0", "Hl": "hl"}
 
 
 class Projector:
     def __init__(self, basis, nx, ny, angles, nu, dx, dy, du, lsd, lso, drho=None):
+        """Square lattices (B0, B1, SF): nx x ny pixels of dx x dy.  For the
+        hexagonal bases use Projector.hexagonal."""
         if basis not in BASES:
             raise ValueError("unknown basis %r" % (basis,))
         if not torch.cuda.is_available():
@@ -31,7 +33,13 @@ class Projector:
         L = _lib.lib()
         torch.cuda.init()
         pre = BASES[basis]
-        if pre == "SF":
+        if basis in ("H0", "Hl"):
+            # hexagonal: nx = nc columns, ny = nl rows, dx = dt (lattice constant)
+            dr = None if drho is None else np.ascontiguousarray(drho, dtype=np.float64)
+            self._drho = dr
+            h = getattr(L, pre + "_init")(self.ny, self.nx, self.np, self.nu, dx, du, lsd, lso, _lib.dptr(ang),
+                                          None, None if dr is None else _lib.dptr(dr))
+        elif pre == "SF":
             if drho is not None:
                 raise ValueError("SF_init takes no drho (layers/A_SF.cu:396)")
             h = L.SF_init(self.nx, self.ny, self.np, self.nu, dx, dy, du, lsd, lso, _lib.dptr(ang))
@@ -44,6 +52,15 @@ class Projector:
             raise _lib.SgdctError("%s_init failed: %s" % (pre, _lib.last_error()))
         self.handle = h
         self.device = torch.device("cuda", torch.cuda.current_device())
+
+    @classmethod
+    def hexagonal(cls, basis, nl, nc, angles, nu, dt, du, lsd, lso, drho=None):
+        """H0 / Hl on nl rows x nc columns (images (nl, nc), [il][ic]); odd
+        rows hold nc-1 points, their last column is ignored by A and zeroed
+        by A^T."""
+        if basis not in ("H0", "Hl"):
+            raise ValueError("hexagonal bases are H0 and Hl")
+        return cls(basis, nc, nl, angles, nu, dt, dt, du, lsd, lso, drho=drho)
 
     def close(self):
         if getattr(self, "handle", None):

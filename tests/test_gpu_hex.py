@@ -1,0 +1,136 @@
+"""GPU parity of the hexagonal-lattice projectors (H0, Hl) against the oracle
+(oracle/sgdct_oracle.c restating layers/A_H0.cu and A_Hl.cu, with the H0
+culling and scale fixes), adjoint test, host ABI, and the autograd checks of
+the hexagonal workload (A as torch.autograd.Function, backward = A^T kernel):
+central finite differences and the adjoint dot-product test on a batch."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+CASES = {
+    "small": (41, 37, 2 * np.pi * np.arange(48) / 48 + 0.02, 96, 2.0, 3.2, 1000.0, 500.3),
+    "degenerate": (31, 27, np.array([0.0, np.pi / 6, np.pi / 3, np.pi / 2, np.pi, 4 * np.pi / 3, 0.4]), 80, 2.4,
+                   3.0, 1000.0, 500.3),
+    "offdetector": (41, 37, 2 * np.pi * np.arange(30) / 30, 20, 2.0, 3.0, 1000.0, 500.3),
+}
+
+
+def lattice_image(nl, nc, seed):
+    rng = np.random.default_rng(seed)
+    img = rng.random((nl, nc))
+    img[1::2, -1] = 0.0
+    return img
+
+
+def make(basis, case):
+    from pyoracle import HexGeometry, Oracle
+    from sgd_amd.torch_ops import Projector
+    nl, nc, ang, nu, dt, du, lsd, lso = CASES[case]
+    if basis == "Hl":
+        nl = 2 * nl - 1  # Hl rows are dt/2 apart
+    print("case %s basis %s nl=%d nc=%d np=%d nu=%d" % (case, basis, nl, nc, len(ang), nu), flush=True)
+    P = Projector.hexagonal(basis, nl, nc, ang, nu, dt, du, lsd, lso)
+    return P, HexGeometry(Oracle("exact"), nl, nc, ang, nu, dt, du, lsd, lso), HexGeometry(
+        Oracle("ref"), nl, nc, ang, nu, dt, du, lsd, lso)
+
+
+def run(P, arr, fwd=True):
+    import torch
+    t = torch.tensor(arr, dtype=torch.float32, device="cuda")
+    out = P.forward(t) if fwd else P.backward(t)
+    torch.cuda.synchronize()
+    return out.double().cpu().numpy()
+
+
+@pytest.mark.parametrize("basis", ["H0", "Hl"])
+@pytest.mark.parametrize("case", ["small", "degenerate", "offdetector"])
+def test_hex_parity(basis, case):
+    P, ge, gr = make(basis, case)
+    img = lattice_image(ge.nl, ge.nc, 1)
+    ya, ye, yr = run(P, img), ge.forward(basis, img), gr.forward(basis, img)
+    print("A   gpu-exact %.2e gpu-ref %.2e ref-exact %.2e" % (rel(ya, ye), rel(ya, yr), rel(yr, ye)))
+    assert rel(ya, ye) <= TOL
+    sino = ye + 0.05 * np.random.default_rng(2).standard_normal(ye.shape)
+    xa, xe = run(P, sino, fwd=False), ge.backward(basis, sino)
+    print("A^T gpu-exact %.2e" % rel(xa, xe))
+    assert rel(xa, xe) <= TOL
+    assert np.all(xa[1::2, -1] == 0.0)
+
+
+@pytest.mark.parametrize("basis", ["H0", "Hl"])
+def test_hex_adjoint_and_host_abi(basis):
+    from sgd_amd import operators as ops
+    P, ge, gr = make(basis, "small")
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((ge.nl, ge.nc))
+    x[1::2, -1] = 0
+    y = rng.standard_normal((ge.np, ge.nu))
+    Ax, Aty = run(P, x), run(P, y, fwd=False)
+    err = abs(np.vdot(Ax, y) - np.vdot(x, Aty)) / (np.linalg.norm(Ax) * np.linalg.norm(y))
+    print("adjoint %s %.2e" % (basis, err))
+    assert err <= 1e-6
+    nl, nc, ang, nu, dt, du, lsd, lso = CASES["small"]
+    cls = ops.A_H0_operator if basis == "H0" else ops.A_Hl_operator
+    op = cls(ge.nl, ge.nc, len(ang), nu, dt, du, lsd, lso, ang, None, None)
+    b = op.matvec(x.T.ravel())  # host layout x[ic*nl + il]
+    assert rel(b.reshape(ge.np, ge.nu), ge.forward(basis, x)) <= TOL
+    xt = op.rmatvec(y.ravel()).reshape(ge.nc, ge.nl).T
+    assert rel(xt, ge.backward(basis, y)) <= TOL
+
+
+@pytest.mark.parametrize("basis", ["H0", "Hl"])
+def test_autograd_hexagonal_batch(basis):
+    """Config-3 pattern on a reduced lattice: batch of 64 hexagonal phantoms,
+    A as an autograd Function; backward (A^T kernel) checked by central finite
+    differences (h = 1e-3 on 256 random lattice points, FP64 loss sums) and by
+    the adjoint dot-product test with Gaussian x, y."""
+    import torch
+    from sgd_amd.torch_ops import Projector, project
+    nl, nc = (75, 64) if basis == "H0" else (149, 64)
+    ang = 2 * np.pi * np.arange(256) / 256
+    print("autograd %s nl=%d nc=%d batch=64" % (basis, nl, nc), flush=True)
+    P = Projector.hexagonal(basis, nl, nc, ang, 256, 2.0, 1.6, 1000.0, 500.3)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.rand((64, nl, nc), device="cuda", generator=gen)
+    x[:, 1::2, -1] = 0
+    w = torch.randn((64, P.np, P.nu), device="cuda", generator=gen)
+    xr = x.clone().requires_grad_(True)
+    loss = (project(xr, P) * w).sum()
+    loss.backward()
+    g = xr.grad
+    rng = np.random.default_rng(0)
+    h = 1e-3
+    idx = []
+    while len(idx) < 256:
+        b_, i_, j_ = rng.integers(64), rng.integers(nl), rng.integers(nc)
+        if not (i_ & 1 and j_ == nc - 1):
+            idx.append((b_, i_, j_))
+    errs = []
+    with torch.no_grad():
+        for (b_, i_, j_) in idx:
+            xp = x[b_].clone(); xp[i_, j_] += h
+            xm = x[b_].clone(); xm[i_, j_] -= h
+            lp = (P.forward(xp).double() * w[b_].double()).sum()
+            lm = (P.forward(xm).double() * w[b_].double()).sum()
+            fd = float((lp - lm) / (2 * h))
+            errs.append(abs(fd - float(g[b_, i_, j_])))
+    scale = float(g.abs().mean())
+    print("FD max abs err %.3e (mean |grad| %.3e)" % (max(errs), scale))
+    assert max(errs) <= 2e-2 * scale
+    # adjoint dot-product test on the batch
+    xg = torch.randn((64, nl, nc), device="cuda", generator=gen)
+    xg[:, 1::2, -1] = 0
+    yg = torch.randn((64, P.np, P.nu), device="cuda", generator=gen)
+    lhs = (P.forward(xg).double() * yg.double()).sum()
+    rhs = (xg.double() * P.backward(yg).double()).sum()
+    err = float(abs(lhs - rhs) / (P.forward(xg).double().norm() * yg.double().norm()))
+    print("batch adjoint %.2e" % err)
+    assert err <= 1e-6
