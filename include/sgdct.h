@@ -92,6 +92,20 @@ int sgdct_shape(const sgdct_projector *h, int *out);
  * count layers/A_B1.cu:643 returns), as 64-bit; negative on error. */
 long long sgdct_eval_count(sgdct_projector *h);
 
+/* Iterative-reconstruction kernels (device pointers, FP32, asynchronous on
+ * `stream`).  No reference counterpart: optimizer/asd_pocs.cu is empty; these
+ * implement the per-pixel steps of SIRT and ASD-POCS (Sidky & Pan 2008).
+ *   tv_gradient:   g = d/dx sum sqrt(eps + dx^2 + dy^2), backward differences
+ *   sirt_residual: r = w * (b - ax), w broadcast over n / per batch entries
+ *   sirt_update:   x = x + lam * v * c, then max(x, 0) if positivity
+ *   axpy:          x = x + a * g */
+int sgdct_tv_gradient(const float *x, float *g, int nx, int ny, int batch, float eps, void *stream);
+int sgdct_sirt_residual(const float *b, const float *ax, const float *w, float *r, long long n,
+                        long long per, void *stream);
+int sgdct_sirt_update(float *x, const float *c, const float *v, float lam, int positivity, long long n,
+                      long long per, void *stream);
+int sgdct_axpy(float *x, const float *g, float a, long long n, void *stream);
+
 /* Human-readable description of the calling thread's last error. */
 const char *sgdct_last_error(void);
 

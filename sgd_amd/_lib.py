@@ -65,6 +65,13 @@ def lib():
     L.sgdct_shape.restype = ctypes.c_int
     L.sgdct_eval_count.argtypes = [_h]
     L.sgdct_eval_count.restype = ctypes.c_longlong
+    L.sgdct_tv_gradient.argtypes = [_fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp]
+    L.sgdct_sirt_residual.argtypes = [_fp, _fp, _fp, _fp, ctypes.c_longlong, ctypes.c_longlong, _fp]
+    L.sgdct_sirt_update.argtypes = [_fp, _fp, _fp, ctypes.c_float, ctypes.c_int, ctypes.c_longlong,
+                                    ctypes.c_longlong, _fp]
+    L.sgdct_axpy.argtypes = [_fp, _fp, ctypes.c_float, ctypes.c_longlong, _fp]
+    for name in ("sgdct_tv_gradient", "sgdct_sirt_residual", "sgdct_sirt_update", "sgdct_axpy"):
+        getattr(L, name).restype = ctypes.c_int
     L.sgdct_last_error.argtypes = []
     L.sgdct_last_error.restype = ctypes.c_char_p
     _lib = L

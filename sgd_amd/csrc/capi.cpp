@@ -18,8 +18,9 @@
 namespace sgdct {
 hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, float* sino, int batch,
                           hipStream_t s);
-hipError_t launch_backward(int basis, const Proj2DArgs& g, const float* sino, float* img, int batch,
+hipError_t launch_backward(int basis, const Proj2DArgs& g, bool use_lds, const float* sino, float* img, int batch,
                            hipStream_t s);
+int max_tile_window(const Proj2DArgs& g, const ViewGeo* views_host);
 hipError_t launch_count(const Proj2DArgs& g, unsigned long long* out, hipStream_t s);
 hipError_t launch_sf_forward(const Proj2DArgs& g, const float* img, float* sino, int batch, hipStream_t s);
 hipError_t launch_sf_backward(const Proj2DArgs& g, const float* sino, float* img, int batch, hipStream_t s);
@@ -30,6 +31,11 @@ hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const f
 hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const float* sino, float* img, int batch,
                                hipStream_t s);
 hipError_t launch_hex_count(const Proj2DArgs& g, double dt, unsigned long long* out, hipStream_t s);
+hipError_t launch_tv_grad(const float* x, float* g, int nx, int ny, int batch, float eps, hipStream_t s);
+hipError_t launch_resid(const float* b, const float* ax, const float* w, float* r, size_t n, size_t per, hipStream_t s);
+hipError_t launch_update(float* x, const float* c, const float* v, float lam, int positivity, size_t n, size_t per,
+                         hipStream_t s);
+hipError_t launch_axpy(float* x, const float* g, float a, size_t n, hipStream_t s);
 }  // namespace sgdct
 
 using namespace sgdct;
@@ -84,6 +90,7 @@ struct sgdct_projector {
   float* h_sino = nullptr;  // pinned
   unsigned long long* d_count = nullptr;
   long long evals = -1;
+  bool bwd_lds = false;  // A^T stages bin windows in LDS (B0 / B1)
   Proj2DArgs args{};
 
   ~sgdct_projector() {
@@ -217,6 +224,8 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   a.dx = dx; a.dy = dy;
   a.h = h->h;
   a.nchunk = (nu + 63) / 64;
+  // 2 bins of slack for FP64 contraction differences between host and device
+  if (basis == BASIS_B0 || basis == BASIS_B1) h->bwd_lds = max_tile_window(a, views.data()) <= kMaxWindow - 2;
   return h;
 }
 
@@ -243,7 +252,7 @@ int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, h
   hipError_t e;
   if (is_hex(h->basis)) e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
   else if (h->basis == BASIS_SF) e = launch_sf_backward(h->args, sino, img, batch, s);
-  else e = launch_backward(h->basis, h->args, sino, img, batch, s);
+  else e = launch_backward(h->basis, h->args, h->bwd_lds, sino, img, batch, s);
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "backward launch");
 }
 
@@ -387,6 +396,32 @@ int sgdct_shape(const sgdct_projector* h, int* out) {
 long long sgdct_eval_count(sgdct_projector* h) {
   if (!h) return fail(SGDCT_EINVAL, "null handle");
   return eval_count(h);
+}
+
+int sgdct_tv_gradient(const float* x, float* g, int nx, int ny, int batch, float eps, void* stream) {
+  if (!x || !g || nx <= 0 || ny <= 0 || batch <= 0 || !(eps > 0)) return fail(SGDCT_EINVAL, "bad tv_gradient arguments");
+  hipError_t e = launch_tv_grad(x, g, nx, ny, batch, eps, (hipStream_t)stream);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, "tv_gradient launch");
+}
+
+int sgdct_sirt_residual(const float* b, const float* ax, const float* w, float* r, long long n, long long per,
+                        void* stream) {
+  if (!b || !ax || !w || !r || n <= 0 || per <= 0 || n % per) return fail(SGDCT_EINVAL, "bad sirt_residual arguments");
+  hipError_t e = launch_resid(b, ax, w, r, (size_t)n, (size_t)per, (hipStream_t)stream);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, "sirt_residual launch");
+}
+
+int sgdct_sirt_update(float* x, const float* c, const float* v, float lam, int positivity, long long n, long long per,
+                      void* stream) {
+  if (!x || !c || !v || n <= 0 || per <= 0 || n % per) return fail(SGDCT_EINVAL, "bad sirt_update arguments");
+  hipError_t e = launch_update(x, c, v, lam, positivity, (size_t)n, (size_t)per, (hipStream_t)stream);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, "sirt_update launch");
+}
+
+int sgdct_axpy(float* x, const float* g, float a, long long n, void* stream) {
+  if (!x || !g || n <= 0) return fail(SGDCT_EINVAL, "bad axpy arguments");
+  hipError_t e = launch_axpy(x, g, a, (size_t)n, (hipStream_t)stream);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, "axpy launch");
 }
 
 const char* sgdct_last_error(void) { return g_last_error.c_str(); }

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float*
   const double sq = rows ? qcx : qcy, mq = rows ? qcy : qcx;
   const double sr = rows ? rcx : rcy, mr = rows ? rcy : rcx;
   const int nmin = rows ? g.nx : g.ny, nmaj = rows ? g.ny : g.nx;
-  const size_t smin = rows ? 1 : g.nx, smaj = rows ? g.nx : 1;
+  const int smin = rows ? 1 : g.nx, smaj = rows ? g.nx : 1;
   const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
   const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
 
@@ -110,30 +110,48 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float*
     m_hi = (int)hi;
   }
 
+  // Pixel interval of line m: [ceil(lo(m)), floor(hi(m))], lo and hi affine
+  // in m (FP32).  sq and sr share a sign whenever the strip is within ~45
+  // degrees of the walk direction, which the per-wave orientation guarantees;
+  // otherwise the lane walks whole lines (outside the corner range B1's
+  // footprint is exactly zero).  Exact integer ties are not special-cased:
+  // the corner rule itself is discontinuous there.
+  float lo0 = 0.0f, lo1 = 0.0f, hi0 = (float)(nmin - 1), hi1 = 0.0f;
+  if (sq > 0.0 && sr > 0.0) {
+    lo0 = (float)((-ra - qc0) * isq);
+    lo1 = (float)(-mq * isq);
+    hi0 = (float)((rb - rc0) * isr);
+    hi1 = (float)(-mr * isr);
+  } else if (sq < 0.0 && sr < 0.0) {
+    hi0 = (float)((-ra - qc0) * isq);
+    hi1 = (float)(-mq * isq);
+    lo0 = (float)((rb - rc0) * isr);
+    lo1 = (float)(-mr * isr);
+  }
+  // ta, tb at a line's first pixel in FP64: ta = ka N_u = an * n + am * m + a0
+  const double an = (double)rec.ka * sa, am = (double)rec.ka * ma, a0 = (double)rec.ka * ac0;
+  const double bn = (double)rec.kb * sb, bm = (double)rec.kb * mb, b0 = (double)rec.kb * bc0;
+  const float lastn = (float)(nmin - 1);
+
   double acc = 0.0;
   for (int m = m_lo; m <= m_hi; ++m) {
-    const double r1 = -ra - mq * m - qc0;  // need sq * n >= r1
-    const double r2 = rb - mr * m - rc0;   // need sr * n <  r2
-    double lo = 0.0, hi = (double)(nmin - 1);
-    if (sq > 0.0) lo = fmax(lo, ceil(r1 * isq));
-    else if (sq < 0.0) hi = fmin(hi, floor(r1 * isq));
-    else if (r1 > 0.0) continue;
-    if (sr > 0.0) hi = fmin(hi, ceil(r2 * isr) - 1.0);
-    else if (sr < 0.0) lo = fmax(lo, floor(r2 * isr) + 1.0);
-    else if (r2 <= 0.0) continue;
-    if (lo > hi) continue;
-    const int n0 = (int)lo, n1 = (int)hi;
-    float ta = (float)((double)rec.ka * (sa * n0 + ma * m + ac0));
-    float tb = (float)((double)rec.kb * (sb * n0 + mb * m + bc0));
-    const float* p = img + (size_t)m * smaj + (size_t)n0 * smin;
+    const float fm = (float)m;
+    const float flo = fmaxf(ceilf(__builtin_fmaf(fm, lo1, lo0)), 0.0f);
+    const float fhi = fminf(floorf(__builtin_fmaf(fm, hi1, hi0)), lastn);
+    if (flo > fhi) continue;
+    const int n0 = (int)flo;
+    const double dm = (double)m, dn = (double)n0;
+    float ta = (float)__builtin_fma(dn, an, __builtin_fma(dm, am, a0));
+    float tb = (float)__builtin_fma(dn, bn, __builtin_fma(dm, bm, b0));
+    int idx = m * smaj + n0 * smin;
+    const int end = idx + ((int)fhi - n0 + 1) * smin;
     float accl = 0.0f;
-    for (int n = n0; n <= n1; ++n) {
-      const float c = T::conv(rec, ta, tb);
-      accl = __builtin_fmaf(c, *p, accl);
-      p += smin;
+    do {
+      accl = __builtin_fmaf(T::conv(rec, ta, tb), img[idx], accl);
+      idx += smin;
       ta += dta;
       tb += dtb;
-    }
+    } while (idx != end);
     acc += (double)accl;
   }
   sino[(size_t)v * g.nu + u] = (float)(acc * g.dx * g.dy);
@@ -145,8 +163,8 @@ __global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float*
 
 // Bin window of one view for a pixel tile: bins touched by the tile's box
 // widened by h, from the exact projection of its four corners, +-1 margin.
-__device__ inline void tile_window(const Proj2DArgs& g, const ViewGeo& vg, int ix0, int iy0, int* lo,
-                                   int* n) {
+__host__ __device__ inline void tile_window(const Proj2DArgs& g, const ViewGeo& vg, int ix0, int iy0, int* lo,
+                                            int* n, int cap = kMaxWindow) {
   double umin = 1e300, umax = -1e300;
   for (int c = 0; c < 4; ++c) {
     double fx = (c & 1) ? (ix0 + kTile - 1 + g.h) : (ix0 - g.h);
@@ -161,27 +179,40 @@ __device__ inline void tile_window(const Proj2DArgs& g, const ViewGeo& vg, int i
   double l = fmax(floor(umin) - 1.0, 0.0), hgh = fmin(floor(umax) + 1.0, (double)(g.nu - 1));
   int li = (int)l, hi = (int)hgh;
   *lo = li;
-  *n = hi >= li ? min(hi - li + 1, kMaxWindow) : 0;
+  *n = hi >= li ? min(hi - li + 1, cap) : 0;
 }
 
-template <int BASIS>
-__device__ __forceinline__ typename Traits<BASIS>::Bin load_rec(const float4* r) {
-  typename Traits<BASIS>::Bin b;
-  float4* d = reinterpret_cast<float4*>(&b);
-#pragma unroll
-  for (int k = 0; k < (int)(sizeof(b) / 16); ++k) d[k] = r[k];
-  return b;
+// Widest bin window over every tile and view (host-side, on the host copy of
+// the view table): the LDS-staged A^T kernel is used when it is <= kMaxWindow.
+int max_tile_window(const Proj2DArgs& g, const ViewGeo* views_host) {
+  int w = 0;
+  for (int v = 0; v < g.np; ++v)
+    for (int iy0 = 0; iy0 < g.ny; iy0 += kTile)
+      for (int ix0 = 0; ix0 < g.nx; ix0 += kTile) {
+        int lo, n;
+        tile_window(g, views_host[v], ix0, iy0, &lo, &n, 1 << 30);
+        w = n > w ? n : w;
+      }
+  return w;
 }
 
-template <int BASIS>
+// Bin records are staged in LDS with the sinogram value in their first
+// padding word (B1Bin::pad0 / B0Bin::pad0).
+__device__ __forceinline__ void set_y(B1Bin& r, float y) { r.pad0 = y; }
+__device__ __forceinline__ void set_y(B0Bin& r, float y) { r.pad0 = y; }
+__device__ __forceinline__ float get_y(const B1Bin& r) { return r.pad0; }
+__device__ __forceinline__ float get_y(const B0Bin& r) { return r.pad0; }
+
+// USE_LDS: the host has checked that every tile's bin window fits in
+// kMaxWindow bins for every view (projector creation); otherwise records are
+// read from global memory (L2) directly.
+template <int BASIS, bool USE_LDS>
 __global__ __launch_bounds__(256) void bwd_pix_kernel(Proj2DArgs g, const float* __restrict__ sino,
                                                       float* __restrict__ img) {
   using T = Traits<BASIS>;
   using Bin = typename T::Bin;
-  constexpr int BIN4 = sizeof(Bin) / 16;
   static_assert(sizeof(Bin) % 16 == 0, "bin records are whole float4s");
-  __shared__ float4 lds[kViewChunk][kMaxWindow][BIN4];
-  __shared__ float ldsy[kViewChunk][kMaxWindow];
+  __shared__ Bin lds[USE_LDS ? kViewChunk : 1][USE_LDS ? kMaxWindow : 1];
   __shared__ int win_lo[kViewChunk], win_n[kViewChunk];
 
   const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
@@ -191,70 +222,78 @@ __global__ __launch_bounds__(256) void bwd_pix_kernel(Proj2DArgs g, const float*
   img += (size_t)blockIdx.z * g.nx * g.ny;
   const bool valid = ix < g.nx && iy < g.ny;
   const Bin* bins = reinterpret_cast<const Bin*>(g.bins);
+  const double fx = ix * g.dx, fy = iy * g.dy;
+  const float last = (float)(g.nu - 1);
 
   double acc = 0.0;
   for (int v0 = 0; v0 < g.np; v0 += kViewChunk) {
     const int nv = min(kViewChunk, g.np - v0);
-    if ((int)threadIdx.x < nv) {
-      int lo, n;
-      tile_window(g, g.views[v0 + threadIdx.x], ix0, iy0, &lo, &n);
-      win_lo[threadIdx.x] = lo;
-      win_n[threadIdx.x] = n;
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nv * kMaxWindow; idx += blockDim.x) {
-      const int j = idx / kMaxWindow, i = idx - j * kMaxWindow;
-      if (i < win_n[j]) {
-        const size_t gi = (size_t)(v0 + j) * g.nu + win_lo[j] + i;
-        const float4* src = reinterpret_cast<const float4*>(bins + gi);
-#pragma unroll
-        for (int k = 0; k < BIN4; ++k) lds[j][i][k] = src[k];
-        ldsy[j][i] = sino[gi];
+    if (USE_LDS) {
+      if ((int)threadIdx.x < nv) {
+        int lo, n;
+        tile_window(g, g.views[v0 + threadIdx.x], ix0, iy0, &lo, &n);
+        win_lo[threadIdx.x] = lo;
+        win_n[threadIdx.x] = n;
       }
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < nv * kMaxWindow; idx += blockDim.x) {
+        const int j = idx / kMaxWindow, i = idx - j * kMaxWindow;
+        if (i < win_n[j]) {
+          const size_t gi = (size_t)(v0 + j) * g.nu + win_lo[j] + i;
+          Bin r = bins[gi];
+          set_y(r, sino[gi]);
+          lds[j][i] = r;
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
     if (valid) {
       for (int j = 0; j < nv; ++j) {
         const int v = v0 + j;
         const ViewGeo& vg = g.views[v];
-        const double wx = ix * g.dx + vg.w0x, wy = iy * g.dy + vg.w0y;
+        const double wx = fx + vg.w0x, wy = fy + vg.w0y;
         const double n0 = vg.e0x * wy - vg.e0y * wx;
         const double be = vg.pux * wy - vg.puy * wx;
         const float bef = (float)be;
-        const float ibe = __builtin_amdgcn_rcpf(bef);
-        const float kc = -(float)n0 * ibe;
+        const float kc = -(float)n0 * __builtin_amdgcn_rcpf(bef);
         const float gx = __builtin_fmaf(kc, vg.gx1, vg.gx0), gy = __builtin_fmaf(kc, vg.gy1, vg.gy0);
         const float hw = g.h * (__builtin_fabsf(gx) + __builtin_fabsf(gy)) *
                          __builtin_amdgcn_rcpf(__builtin_fabsf(bef) - vg.db);
         // clamped corner range (layers/A_B1.cu:150-151): pixels projecting
         // wholly before / past the detector still visit bin 0 / nu-1
-        const float last = (float)(g.nu - 1);
-        const float flo = fminf(fmaxf(floorf(kc - hw), 0.0f), last);
-        const float fhi = fminf(fmaxf(floorf(kc + hw), 0.0f), last);
-        const int ulo = (int)flo, uhi = (int)fhi;
-        float na = (float)(n0 + (double)ulo * be);
-        const int wlo = win_lo[j], wn = win_n[j];
+        float flo = fminf(fmaxf(floorf(kc - hw), 0.0f), last);
+        float fhi = fminf(fmaxf(floorf(kc + hw), 0.0f), last);
+        if (USE_LDS) {
+          // The window holds every bin of the exact corner range of the
+          // tile (+-1); bins of the rounded range outside it carry a zero
+          // footprint, so clamping keeps the LDS reads inside the window.
+          flo = fmaxf(flo, (float)win_lo[j]);
+          fhi = fminf(fhi, (float)(win_lo[j] + win_n[j] - 1));
+        }
+        const int ulo = (int)flo, cnt = (int)fhi - ulo + 1;
+        float na = (float)__builtin_fma((double)ulo, be, n0);
         float accv = 0.0f;
-        for (int u = ulo; u <= uhi; ++u) {
-          const int wi = u - wlo;
-          Bin rec;
-          float yv;
-          if (wi >= 0 && wi < wn) {
-            rec = load_rec<BASIS>(&lds[j][wi][0]);
-            yv = ldsy[j][wi];
-          } else {
-            rec = bins[(size_t)v * g.nu + u];
-            yv = sino[(size_t)v * g.nu + u];
+        if (USE_LDS) {
+          const Bin* rp = &lds[j][ulo - win_lo[j]];
+          for (int k = 0; k < cnt; ++k) {
+            const Bin rec = rp[k];
+            const float nb = na + bef;
+            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), get_y(rec), accv);
+            na = nb;
           }
-          const float nb = na + bef;
-          const float ta = rec.ka * na, tb = rec.kb * nb;
-          accv = __builtin_fmaf(T::conv(rec, ta, tb), yv, accv);
-          na = nb;
+        } else {
+          const size_t base = (size_t)v * g.nu + ulo;
+          for (int k = 0; k < cnt; ++k) {
+            const Bin rec = bins[base + k];
+            const float nb = na + bef;
+            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), sino[base + k], accv);
+            na = nb;
+          }
         }
         acc += (double)accv;
       }
     }
-    __syncthreads();
+    if (USE_LDS) __syncthreads();
   }
   if (valid) img[(size_t)iy * g.nx + ix] = (float)(acc * g.dx * g.dy);
 }
@@ -304,12 +343,18 @@ hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, floa
   return hipGetLastError();
 }
 
-hipError_t launch_backward(int basis, const Proj2DArgs& g, const float* sino, float* img, int batch,
+hipError_t launch_backward(int basis, const Proj2DArgs& g, bool use_lds, const float* sino, float* img, int batch,
                            hipStream_t s) {
   dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, batch), block(256);
-  if (basis == BASIS_B1) bwd_pix_kernel<BASIS_B1><<<grid, block, 0, s>>>(g, sino, img);
-  else if (basis == BASIS_B0) bwd_pix_kernel<BASIS_B0><<<grid, block, 0, s>>>(g, sino, img);
-  else return hipErrorInvalidValue;
+  if (basis == BASIS_B1) {
+    if (use_lds) bwd_pix_kernel<BASIS_B1, true><<<grid, block, 0, s>>>(g, sino, img);
+    else bwd_pix_kernel<BASIS_B1, false><<<grid, block, 0, s>>>(g, sino, img);
+  } else if (basis == BASIS_B0) {
+    if (use_lds) bwd_pix_kernel<BASIS_B0, true><<<grid, block, 0, s>>>(g, sino, img);
+    else bwd_pix_kernel<BASIS_B0, false><<<grid, block, 0, s>>>(g, sino, img);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
