@@ -77,6 +77,14 @@ class Oracle:
             lib.or_hex_strip_exact.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 4
             lib.or_hl_conv.restype = ctypes.c_double
             lib.or_hl_conv.argtypes = [ctypes.c_double] * 5
+            lib.or_geo3_create.restype = ctypes.c_void_p
+            lib.or_geo3_create.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double] * 7 + [_dp]
+            lib.or_geo3_destroy.argtypes = [ctypes.c_void_p]
+            for name in ("or_b1_3d_forward", "or_b1_3d_backward"):
+                f = getattr(lib, name)
+                f.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int]
+            lib.or_b1_3d_cell_exact.restype = ctypes.c_double
+            lib.or_b1_3d_cell_exact.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7
             Oracle._libs[path] = lib
         self.lib = Oracle._libs[path]
         self.mode = mode
@@ -213,3 +221,44 @@ class HexGeometry(Geometry):
         cy = (il - 0.5 * (self.nl - 1)) * dy
         cx = (ic - 0.5 * (self.nc - (2 if il & 1 else 1))) * dx
         return cx, cy
+
+
+class Geometry3:
+    """Cone-beam geometry of layers/A_B1_3D.cu: nx x ny x nz voxels, an nu x nv
+    flat detector.  Volumes are (nz, ny, nx) arrays [iz, iy, ix]; projections
+    (np, nv, nu) [p, j, u]."""
+
+    def __init__(self, oracle, nx, ny, nz, angles, nu, nv, dx, dy, dz, du, dv, lsd, lso):
+        self.o = oracle
+        self.lib = oracle.lib
+        self.nx, self.ny, self.nz, self.nu, self.nv = nx, ny, nz, nu, nv
+        self.angles = np.ascontiguousarray(angles, dtype=np.float64)
+        self.np = self.angles.size
+        self.d = (dx, dy, dz, du, dv, lsd, lso)
+        self.h = self.lib.or_geo3_create(nx, ny, nz, self.np, nu, nv, dx, dy, dz, du, dv, lsd, lso,
+                                         _ptr(self.angles))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.or_geo3_destroy(self.h)
+            self.h = None
+
+    def _views(self, views):
+        return (0, self.np) if views is None else views
+
+    def forward(self, vol, views=None):
+        vol = np.ascontiguousarray(vol, dtype=np.float64).reshape(self.nz, self.ny, self.nx)
+        out = np.zeros((self.np, self.nv, self.nu))
+        p0, p1 = self._views(views)
+        self.lib.or_b1_3d_forward(self.h, _ptr(vol), _ptr(out), p0, p1)
+        return out
+
+    def backward(self, proj, views=None):
+        proj = np.ascontiguousarray(proj, dtype=np.float64).reshape(self.np, self.nv, self.nu)
+        out = np.zeros((self.nz, self.ny, self.nx))
+        p0, p1 = self._views(views)
+        self.lib.or_b1_3d_backward(self.h, _ptr(proj), _ptr(out), p0, p1)
+        return out
+
+    def cell_exact(self, p, u, j, ix, iy, iz, panels=8):
+        return self.lib.or_b1_3d_cell_exact(self.h, p, u, j, ix, iy, iz, panels)

@@ -1055,4 +1055,291 @@ double or_hex_strip_exact(int basis, const void *h, double dt, int p, int u, int
     return total;
 }
 
+/* ------------------------------------------------------------------------- */
+/* 3-D cone beam, layers/A_B1_3D.cu:82-312: the in-plane B1 footprint of the  */
+/* voxel's (ix, iy) column (the 2-D B1 formulas, A_B1_3D.cu:95-283) times an  */
+/* axial trapezoid in detector rows, a path-length weight and the trapezoid's */
+/* inverse area.  Volume vol[(iz*ny + iy)*nx + ix] (copied raw,               */
+/* A_B1_3D.cu:716); projections proj[(p*nv + j)*nu + u] (A_B1_3D.cu:296,731). */
+/* The source sits in the z = 0 plane (b1_3d_init passes no dz,               */
+/* A_B1_3D.cu:702-703); the detector's v axis is +z with v = nv/2 on that     */
+/* plane (utils/geo.cu:215, 267).                                             */
+/* The reference computes the views at 3pi/4 and 7pi/4 through the singular   */
+/* branch of utils/geo.cu:39-41 and A_B1_3D.cu:162-167, whose (x*1.5 - 1)     */
+/* rescale holds only for nz = nv = 1; the closed-form map used here is the   */
+/* projective map the other views use.  Its back-projector multiplies by the  */
+/* zeroed output volume (A_B1_3D.cu:408, 436) and returns zeros; the          */
+/* or_b1_3d_backward below is the transpose of the forward.                   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    ogeo *g;
+    int nz, nv;
+    double dz, dv;
+    fpt *pmv; /* np*4: index-space v numerator row (ix, iy, iz, 1) */
+} ogeo3;
+
+void *or_geo3_create(int nx, int ny, int nz, int np, int nu, int nv, double dx, double dy, double dz,
+                     double du, double dv, double lsd, double lso, const double *angles) {
+    ogeo3 *t = (ogeo3 *)calloc(1, sizeof(ogeo3));
+    t->g = (ogeo *)or_geo_create(nx, ny, np, nu, dx, dy, du, lsd, lso, angles, 0);
+    t->nz = nz; t->nv = nv; t->dz = dz; t->dv = dv;
+    t->pmv = (fpt *)malloc(sizeof(fpt) * 4 * np);
+    fpt fox = (fpt)((nx - 1) * (float)dx / 2.0f);
+    fpt foy = (fpt)((ny - 1) * (float)dy / 2.0f);
+    fpt foz = (fpt)((nz - 1) * (float)dz / 2.0f);
+    for (int p = 0; p < np; ++p) {
+        const double *src = t->g->src + 2 * p, *dtv = t->g->dtv + 2 * p;
+        double ex = dtv[0] - src[0], ey = dtv[1] - src[1];
+        double L = sqrt(ex * ex + ey * ey);
+        ex /= L;
+        ey /= L;
+        double d0 = -(src[0] * ex + src[1] * ey);
+        double vc = nv / 2.0;
+        /* v = vc + (L/dv) z / ((X - src).e), rows scaled so pm[11] == 1 */
+        double r1[4] = {vc * ex / d0, vc * ey / d0, (L / dv) / d0, vc};
+        fpt *q = t->pmv + 4 * p;
+        q[0] = (fpt)(dx * r1[0]);
+        q[1] = (fpt)(dy * r1[1]);
+        q[2] = (fpt)(dz * r1[2]);
+        q[3] = (fpt)(r1[3] - (double)fox * r1[0] - (double)foy * r1[1] - (double)foz * r1[2]);
+    }
+    return t;
+}
+
+void or_geo3_destroy(void *h) {
+    ogeo3 *t = (ogeo3 *)h;
+    if (!t) return;
+    or_geo_destroy(t->g);
+    free(t->pmv);
+    free(t);
+}
+
+/* In-plane B1 convs of column (ix, iy) over its clamped corner range, no
+ * skip (A_B1_3D.cu:117-151, 261-277). */
+static void b1_column_convs(const ogeo *g, int p, const fpt *vx, const fpt *vy, int ix, int iy,
+                            int *minu_out, int *maxu_out, fpt *conv) {
+    int nx = g->nx, ny = g->ny, nu = g->nu;
+    fpt dx = (fpt)g->dx, dy = (fpt)g->dy;
+    fpt px = (fpt)g->src[2 * p], py = (fpt)g->src[2 * p + 1];
+    fpt pux = (fpt)g->puv[2 * p], puy = (fpt)g->puv[2 * p + 1];
+    fpt dtx = (fpt)g->dtv[2 * p], dty = (fpt)g->dtv[2 * p + 1];
+    fpt us[4];
+    int minu, maxu;
+    fpt ox = ((fpt)ix - (fpt)0.5f * (fpt)(nx - 1)) * dx;
+    fpt oy = ((fpt)iy - (fpt)0.5f * (fpt)(ny - 1)) * dy;
+    corner_range(g->pmi + 12 * p, ix, iy, (fpt)1.0f, nu, us, &minu, &maxu);
+    fpt sx = dtx + pux * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    fpt sy = dty + puy * ((fpt)minu - (fpt)0.5f * (fpt)nu + (fpt)0.5f);
+    for (int u = minu; u <= maxu; ++u, sx += pux, sy += puy) {
+        fpt ax = sx - pux * (fpt)0.5f, ay = sy - puy * (fpt)0.5f;
+        fpt bx = sx + pux * (fpt)0.5f, by = sy + puy * (fpt)0.5f;
+        fpt ra = ((ox - px) * (sx - px) + (oy - py) * (sy - py)) / ((ax - px) * (sx - px) + (ay - py) * (sy - py));
+        fpt rb = ((ox - px) * (sx - px) + (oy - py) * (sy - py)) / ((bx - px) * (sx - px) + (by - py) * (sy - py));
+        fpt ax1 = px + (ax - px) * ra, ay1 = py + (ay - py) * ra;
+        fpt bx1 = px + (bx - px) * rb, by1 = py + (by - py) * rb;
+        fpt x1 = (fpt)sqrt((double)((ax1 - ox) * (ax1 - ox) + (ay1 - oy) * (ay1 - oy)));
+        fpt x2 = (fpt)sqrt((double)((bx1 - ox) * (bx1 - ox) + (by1 - oy) * (by1 - oy)));
+        fpt sgn = ((ax1 - ox) * (bx1 - ox) + (ay1 - oy) * (by1 - oy) > 0) ? (fpt)1.0f : (fpt)-1.0f;
+        conv[u - minu] = b1_conv(x1 * sgn, x2, fp_max(vx[u], vy[u]), fp_min(vx[u], vy[u]));
+    }
+    *minu_out = minu;
+    *maxu_out = maxu;
+}
+
+/* Axial part of voxel (ix, iy, iz): sorted row coordinates of the trapezoid
+ * (x index +-1 at the column's iy, z index +-1/2; A_B1_3D.cu:133-137,
+ * 188-202), the clamped row range, and the factor
+ * dx dy dz * weight * 2 / ((vs3 - vs0) + (vs2 - vs1)) (A_B1_3D.cu:209-210).
+ * Returns 0 when the trapezoid lies past the last row (A_B1_3D.cu:204),
+ * -1 for an empty row range, 1 otherwise. */
+static int b1_3d_axial(const ogeo3 *t, int p, int ix, int iy, int iz, fpt *vs, int *minv, int *maxv,
+                       fpt *cfac) {
+    const ogeo *g = t->g;
+    const fpt *pm = g->pmi + 12 * p, *pv = t->pmv + 4 * p;
+    fpt sx1 = (fpt)ix - (fpt)1.0f, sx2 = (fpt)ix + (fpt)1.0f, fy = (fpt)iy;
+    fpt v1 = pv[0] * sx1 + pv[1] * fy + pv[3];
+    fpt v2 = pv[0] * sx2 + pv[1] * fy + pv[3];
+    fpt v3 = pm[8] * sx1 + pm[9] * fy + pm[11];
+    fpt v4 = pm[8] * sx2 + pm[9] * fy + pm[11];
+    fpt z1 = (fpt)iz - (fpt)0.5f, z2 = (fpt)iz + (fpt)0.5f;
+    vs[0] = (v1 + pv[2] * z1) / v3;
+    vs[1] = (v1 + pv[2] * z2) / v3;
+    vs[2] = (v2 + pv[2] * z1) / v4;
+    vs[3] = (v2 + pv[2] * z2) / v4;
+    /* sort4, layers/spline_conv.hh:24-39 (the network of corner_range) */
+#define OR_SORT2(a, b) do { if ((a) > (b)) { fpt t_ = (b); (b) = (a); (a) = t_; } } while (0)
+    OR_SORT2(vs[0], vs[1]);
+    OR_SORT2(vs[2], vs[3]);
+    OR_SORT2(vs[0], vs[2]);
+    OR_SORT2(vs[1], vs[3]);
+    OR_SORT2(vs[1], vs[2]);
+#undef OR_SORT2
+    int nv = t->nv;
+    if (vs[0] >= (fpt)nv) return 0;
+    *minv = imin(imax(0, fp_floor_int(vs[0])), nv - 1);
+    *maxv = imin(imax(0, fp_floor_int(vs[3])), nv - 1);
+    if (*minv > *maxv) return -1;
+    fpt dx = (fpt)g->dx, dy = (fpt)g->dy, dz = (fpt)t->dz;
+    fpt px = (fpt)g->src[2 * p], py = (fpt)g->src[2 * p + 1];
+    fpt ox = ((fpt)ix - (fpt)0.5f * (fpt)(g->nx - 1)) * dx;
+    fpt oy = ((fpt)iy - (fpt)0.5f * (fpt)(g->ny - 1)) * dy;
+    fpt oz = ((fpt)iz - (fpt)0.5f * (fpt)(t->nz - 1)) * dz;
+    fpt r2 = (ox - px) * (ox - px) + (oy - py) * (oy - py);
+    fpt weight = (fpt)sqrt((double)((r2 + oz * oz) / r2));
+    *cfac = (fpt)((double)dx * (double)dy * (double)dz * (double)weight *
+                  (double)((fpt)2.0f / ((vs[3] - vs[0]) + (vs[2] - vs[1]))));
+    return 1;
+}
+
+void or_b1_3d_forward(const void *h, const double *vol, double *proj, int p0, int p1) {
+    const ogeo3 *t = (const ogeo3 *)h;
+    const ogeo *g = t->g;
+    int nx = g->nx, ny = g->ny, nz = t->nz, nu = g->nu, nv = t->nv;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = p0; p < p1; ++p) {
+        double *pj = proj + (size_t)p * nv * nu;
+        memset(pj, 0, sizeof(double) * (size_t)nv * nu);
+        fpt *vx = (fpt *)malloc(sizeof(fpt) * nu), *vy = (fpt *)malloc(sizeof(fpt) * nu);
+        fpt *conv = (fpt *)malloc(sizeof(fpt) * nu);
+        b1_vectors(g, p, vx, vy);
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                int minu, maxu, minv, maxv;
+                int any = 0;
+                for (int iz = 0; iz < nz && !any; ++iz) any = vol[((size_t)iz * ny + iy) * nx + ix] != 0.0;
+                if (!any) continue;
+                b1_column_convs(g, p, vx, vy, ix, iy, &minu, &maxu, conv);
+                for (int iz = 0; iz < nz; ++iz) {
+                    double x = vol[((size_t)iz * ny + iy) * nx + ix];
+                    fpt vs[4], cf;
+                    int r = b1_3d_axial(t, p, ix, iy, iz, vs, &minv, &maxv, &cf);
+                    if (r == 0) break;
+                    if (r < 0 || x == 0.0) continue;
+                    fpt c = (fpt)(x * (double)cf);
+                    for (int u = minu; u <= maxu; ++u) {
+                        double val = (double)(conv[u - minu] * c);
+                        if (val != val) continue;
+                        for (int j = minv; j <= maxv; ++j) {
+                            fpt gm = sf_gamma((fpt)j, (fpt)j + (fpt)1.0f, vs);
+                            if (gm > EPS7) pj[(size_t)j * nu + u] += val * (double)gm;
+                        }
+                    }
+                }
+            }
+        free(vx); free(vy); free(conv);
+    }
+}
+
+/* Transpose of or_b1_3d_forward (the reference's own back-projector returns
+ * zeros, see above): vol[voxel] = sum over views, bins and rows of
+ * conv * cfac * gamma * proj. */
+void or_b1_3d_backward(const void *h, const double *proj, double *vol, int p0, int p1) {
+    const ogeo3 *t = (const ogeo3 *)h;
+    const ogeo *g = t->g;
+    int nx = g->nx, ny = g->ny, nz = t->nz, nu = g->nu, nv = t->nv, np = g->np;
+    fpt *vxs = (fpt *)malloc(sizeof(fpt) * (size_t)np * nu), *vys = (fpt *)malloc(sizeof(fpt) * (size_t)np * nu);
+    for (int p = p0; p < p1; ++p) b1_vectors(g, p, vxs + (size_t)p * nu, vys + (size_t)p * nu);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int col = 0; col < nx * ny; ++col) {
+        int ix = col % nx, iy = col / nx;
+        fpt *conv = (fpt *)malloc(sizeof(fpt) * nu);
+        for (int iz = 0; iz < nz; ++iz) vol[((size_t)iz * ny + iy) * nx + ix] = 0.0;
+        for (int p = p0; p < p1; ++p) {
+            int minu, maxu, minv, maxv;
+            const double *pj = proj + (size_t)p * nv * nu;
+            b1_column_convs(g, p, vxs + (size_t)p * nu, vys + (size_t)p * nu, ix, iy, &minu, &maxu, conv);
+            for (int iz = 0; iz < nz; ++iz) {
+                fpt vs[4], cf;
+                int r = b1_3d_axial(t, p, ix, iy, iz, vs, &minv, &maxv, &cf);
+                if (r == 0) break;
+                if (r < 0) continue;
+                double acc = 0.0;
+                for (int u = minu; u <= maxu; ++u) {
+                    double cu = (double)conv[u - minu] * (double)cf;
+                    if (cu != cu) continue;
+                    for (int j = minv; j <= maxv; ++j) {
+                        fpt gm = sf_gamma((fpt)j, (fpt)j + (fpt)1.0f, vs);
+                        if (gm > EPS7) acc += cu * (double)gm * pj[(size_t)j * nu + u];
+                    }
+                }
+                vol[((size_t)iz * ny + iy) * nx + ix] += acc;
+            }
+        }
+        free(conv);
+    }
+    free(vxs);
+    free(vys);
+}
+
+/* Line integral of hat(x)hat(y)box(z) (hat half-widths dx, dy; box height
+ * dz) centred at c along P + s e (unit e): the support is an interval, the
+ * integrand piecewise quadratic with kinks where x = cx or y = cy; Simpson
+ * on every piece is exact. */
+static double hat_box_line3(const double *P, const double *e, const double *c, double dx, double dy,
+                            double dz) {
+    double lo = -1e300, hi = 1e300, br[2];
+    int nb = 0;
+    const double half[3] = {dx, dy, 0.5 * dz};
+    for (int k = 0; k < 3; ++k) {
+        if (fabs(e[k]) < 1e-300) {
+            if (fabs(P[k] - c[k]) >= half[k]) return 0.0;
+            continue;
+        }
+        double ta = (c[k] - half[k] - P[k]) / e[k], tb = (c[k] + half[k] - P[k]) / e[k];
+        if (ta > tb) { double tt = ta; ta = tb; tb = tt; }
+        if (ta > lo) lo = ta;
+        if (tb < hi) hi = tb;
+        if (k < 2) br[nb++] = (c[k] - P[k]) / e[k];
+    }
+    if (!(hi > lo)) return 0.0;
+    double pts[4];
+    int n = 0;
+    pts[n++] = lo;
+    for (int i = 0; i < nb; ++i)
+        if (br[i] > lo && br[i] < hi) pts[n++] = br[i];
+    pts[n++] = hi;
+    for (int i = 1; i < n; ++i)
+        for (int j = i; j > 0 && pts[j] < pts[j - 1]; --j) { double tt = pts[j]; pts[j] = pts[j - 1]; pts[j - 1] = tt; }
+    double sum = 0.0;
+    for (int i = 0; i + 1 < n; ++i) {
+        double s0 = pts[i], s1 = pts[i + 1], ss[3] = {s0, 0.5 * (s0 + s1), s1}, f[3];
+        for (int k = 0; k < 3; ++k) {
+            double hx = 1.0 - fabs(P[0] + ss[k] * e[0] - c[0]) / dx;
+            double hy = 1.0 - fabs(P[1] + ss[k] * e[1] - c[1]) / dy;
+            f[k] = (hx > 0 ? hx : 0) * (hy > 0 ? hy : 0);
+        }
+        sum += (s1 - s0) * (f[0] + 4 * f[1] + f[2]) / 6.0;
+    }
+    return sum;
+}
+
+/* FP64 mean over detector cell (u, j) of the line integral of voxel
+ * (ix, iy, iz)'s basis along source -> detector-point rays: composite
+ * Gauss-Legendre, `panels` x `panels` panels of 8 x 8 points. */
+double or_b1_3d_cell_exact(const void *h, int p, int u, int j, int ix, int iy, int iz, int panels) {
+    const ogeo3 *t = (const ogeo3 *)h;
+    const ogeo *g = t->g;
+    double c[3] = {(ix - 0.5 * (g->nx - 1)) * g->dx, (iy - 0.5 * (g->ny - 1)) * g->dy,
+                   (iz - 0.5 * (t->nz - 1)) * t->dz};
+    double P[3] = {g->src[2 * p], g->src[2 * p + 1], 0.0};
+    double pux = g->puv[2 * p], puy = g->puv[2 * p + 1];
+    double total = 0.0;
+    for (int a = 0; a < panels; ++a)
+        for (int qa = 0; qa < 8; ++qa) {
+            double s = (a + 0.5 + 0.5 * GL8x[qa]) / panels, ws = GL8w[qa] * 0.5 / panels;
+            double qx = g->dtv[2 * p] + pux * (u + s - 0.5 * g->nu);
+            double qy = g->dtv[2 * p + 1] + puy * (u + s - 0.5 * g->nu);
+            for (int b = 0; b < panels; ++b)
+                for (int qb = 0; qb < 8; ++qb) {
+                    double tt = (b + 0.5 + 0.5 * GL8x[qb]) / panels, wt = GL8w[qb] * 0.5 / panels;
+                    double qz = t->dv * (j + tt - 0.5 * t->nv);
+                    double e[3] = {qx - P[0], qy - P[1], qz - P[2]};
+                    double nn = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+                    e[0] /= nn; e[1] /= nn; e[2] /= nn;
+                    total += ws * wt * hat_box_line3(P, e, c, g->dx, g->dy, t->dz);
+                }
+        }
+    return total;
+}
+
 int or_precision_bits(void) { return (int)(sizeof(fpt) * 8); }
