@@ -72,14 +72,30 @@ sgdct_projector *hl_init(int nl, int nc, int np, int nu, double dt, double du, d
 int hl_forward_projection(double *b, double *x, sgdct_projector *h);
 int hl_backward_projection(double *b, double *x, sgdct_projector *h);
 
+/* layers/A_B1_3D.hh:24-26 / A_B1_3D.cu:701 (b1_3d_init), :709 (forward),
+ * :744 (backward).  Cone beam, in-plane B1 footprint times an axial
+ * trapezoid; volume x copied raw as [iz][iy][ix] (A_B1_3D.cu:716),
+ * projections b = [p][v][u] (:731).  Forward returns the reference's
+ * evaluation count; a value >= 2^63 is a negative SGDCT_E* code.  The
+ * reference's back-projector returns zeros (A_B1_3D.cu:408, 436 multiply by
+ * the zeroed output); this one is the transpose of the forward and returns
+ * SGDCT_OK or a negative code.  nz <= 512 and nv <= 512. */
+sgdct_projector *b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, double dx, double dy,
+                            double dz, double du, double dv, double lsd, double lso,
+                            const double *angles);
+unsigned long long b1_3d_forward_projection(double *b, double *x, sgdct_projector *h);
+int b1_3d_backward_projection(double *b, double *x, sgdct_projector *h);
+
 /* ---- additions ------------------------------------------------------------ */
 
 /* Release a handle from any *_init (the reference leaks them). */
 void sgdct_destroy(sgdct_projector *h);
 
-/* Image (ny, nx) FP32 row-major [iy][ix] and sinogram (np, nu) FP32 [p][u],
- * both device pointers; `batch` images / sinograms back to back; `stream` is
- * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.
+/* Image (ny, nx) FP32 row-major [iy][ix] and sinogram (np, nu) FP32 [p][u]
+ * (cone beam: volume (nz, ny, nx) and projections (np, nv, nu)), both device
+ * pointers; `batch` images / sinograms back to back; `stream` is
+ * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.  A cone-beam
+ * handle keeps per-handle workspace: its calls must not overlap.
  * Returns SGDCT_OK or a negative code. */
 int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
@@ -87,6 +103,10 @@ int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int ba
 /* Shape query: out[0..3] = nx, ny, np, nu (hexagonal: nc, nl, np, nu);
  * returns the basis id (0 B0, 1 B1, 2 SF, 3 H0, 4 Hl) or a negative code. */
 int sgdct_shape(const sgdct_projector *h, int *out);
+
+/* Shape query for any handle: out[0..5] = nx, ny, nz, np, nu, nv (nz = nv = 1
+ * for the fan-beam bases); returns the basis id (5 = cone-beam B1). */
+int sgdct_shape3(const sgdct_projector *h, int *out);
 
 /* Footprint evaluations of one A under the reference's corner rule (the
  * count layers/A_B1.cu:643 returns), as 64-bit; negative on error. */

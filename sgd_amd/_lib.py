@@ -63,6 +63,14 @@ def lib():
         f.restype = ctypes.c_int
     L.sgdct_shape.argtypes = [_h, ctypes.POINTER(ctypes.c_int)]
     L.sgdct_shape.restype = ctypes.c_int
+    L.sgdct_shape3.argtypes = [_h, ctypes.POINTER(ctypes.c_int)]
+    L.sgdct_shape3.restype = ctypes.c_int
+    L.b1_3d_init.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double] * 7 + [_dp]
+    L.b1_3d_init.restype = _h
+    L.b1_3d_forward_projection.argtypes = [_dp, _dp, _h]
+    L.b1_3d_forward_projection.restype = ctypes.c_ulonglong
+    L.b1_3d_backward_projection.argtypes = [_dp, _dp, _h]
+    L.b1_3d_backward_projection.restype = ctypes.c_int
     L.sgdct_eval_count.argtypes = [_h]
     L.sgdct_eval_count.restype = ctypes.c_longlong
     L.sgdct_tv_gradient.argtypes = [_fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp]

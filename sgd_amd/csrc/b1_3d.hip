@@ -1,0 +1,366 @@
+// b1_3d.hip -- gfx950 kernels of the cone-beam B1 projector (layers/A_B1_3D.cu).
+//
+// The reference's value of voxel (ix, iy, iz) in detector cell (u, j) of view p
+// factors into an in-plane part and an axial part (A_B1_3D.cu:95-305):
+//     conv_p(u; ix, iy)  *  x * cfac_p(ix, iy, iz) * gamma_p(j; ix, iy, iz)
+// conv is the 2-D B1 footprint of the (ix, iy) column (it does not depend on
+// iz: the detector's u axis is in-plane), cfac = dx dy dz * path-length weight
+// * inverse area of the voxel's axial trapezoid, gamma its overlap with row j.
+// So per view A = (sum over columns of conv * Y), with
+//     Y_p[c][j] = sum_iz x[iz][c] * cfac * gamma_j      (column c = iy*nx + ix)
+// and each Y value is shared by the ~4-5 bins that see the column.  A runs as
+//   axial_fwd_kernel   Y for a chunk of views (wave per column and view:
+//                      lanes stage every voxel's trapezoid in LDS, then
+//                      gather by detector row; deterministic)
+//   conv_fwd_kernel    proj[p][j][u] = sum_c conv(u, c) Y[c][j] (wave per
+//                      bin: the bin's strip walk of the 2-D A kernel, lanes
+//                      over rows, no atomics)
+// and A^T, which the reference does not provide (its back-projector multiplies
+// by the zeroed output volume, A_B1_3D.cu:408, 436), as the exact transpose:
+//   conv_bwd_kernel    Yb[c][j] = sum_u conv(u, c) proj[p][j][u]
+//   axial_bwd_kernel   x[iz][c] += cfac * sum_j gamma_j Yb[c][j]
+// Volumes are kept column-major (xT[c][iz]) inside, transposed on the way in
+// and out, so a wave's voxel reads are contiguous.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "footprint.hpp"
+#include "projector.hpp"
+
+namespace sgdct {
+
+constexpr int kMaxNz = 512;        // voxels per column staged in LDS
+constexpr int kMaxRowChunks = 8;   // detector rows per lane: nv <= 512
+constexpr float kGammaEps = 1e-7f; // A_B1_3D.cu:300 skips gamma <= eps
+
+// Axial trapezoid of voxel iz in a column: rows of the four corners
+// (x index +-1 at the column's iy, z index +-1/2; A_B1_3D.cu:133-137,
+// 188-202), sorted, and cfac (A_B1_3D.cu:209-210).
+struct Trap {
+  float v0, v1, v2, v3, cf;
+};
+
+struct ColumnAxial {
+  float ka, kb;   // ldz / depth at x index ix - 1 and ix + 1
+  float inv_r2;   // 1 / |in-plane source -> centre|^2
+};
+
+__device__ __forceinline__ ColumnAxial column_axial(const Cone3DArgs& a, const ViewGeo& vg, int ix, int iy) {
+  const double wy = iy * a.g.dy + vg.w0y;
+  const double wxa = (ix - 1) * a.g.dx + vg.w0x, wxb = (ix + 1) * a.g.dx + vg.w0x;
+  const double wx = ix * a.g.dx + vg.w0x;
+  // depth along the central axis = -cross(pu, X - p) / |pu|
+  const double da = -(vg.pux * wy - vg.puy * wxa) / a.du;
+  const double db = -(vg.pux * wy - vg.puy * wxb) / a.du;
+  ColumnAxial c;
+  c.ka = (float)(a.ldz / da);
+  c.kb = (float)(a.ldz / db);
+  c.inv_r2 = (float)(1.0 / (wx * wx + wy * wy));
+  return c;
+}
+
+__device__ __forceinline__ Trap voxel_trap(const Cone3DArgs& a, const ColumnAxial& c, int iz) {
+  const float zc = (float)iz - 0.5f * (float)(a.nz - 1);
+  const float a1 = __builtin_fmaf(c.ka, zc - 0.5f, a.vc), a2 = __builtin_fmaf(c.ka, zc + 0.5f, a.vc);
+  const float b1 = __builtin_fmaf(c.kb, zc - 0.5f, a.vc), b2 = __builtin_fmaf(c.kb, zc + 0.5f, a.vc);
+  Trap t;
+  t.v0 = fminf(a1, b1);
+  t.v3 = fmaxf(a2, b2);
+  const float h1 = fmaxf(a1, b1), l2 = fminf(a2, b2);
+  t.v1 = fminf(h1, l2);
+  t.v2 = fmaxf(h1, l2);
+  const float oz = zc * a.dz;
+  const float w = __builtin_sqrtf(__builtin_fmaf(oz * oz, c.inv_r2, 1.0f));
+  t.cf = a.cvol * w * 2.0f / ((t.v3 - t.v0) + (t.v2 - t.v1));
+  return t;
+}
+
+// Overlap of [s, s+1) with the unit-height trapezoid (A_SF.cu:48-84 /
+// A_B1_3D.cu:30-66), segment by segment.
+__device__ __forceinline__ float trap_gamma(float s, const Trap& t) {
+  const float e = s + 1.0f;
+  float b1 = fmaxf(s, t.v0), b2 = fminf(e, t.v1);
+  float g0 = (b2 > b1) ? (b2 - b1) * (b2 + b1 - 2.0f * t.v0) / (2.0f * (t.v1 - t.v0)) : 0.0f;
+  b1 = fmaxf(s, t.v1);
+  b2 = fminf(e, t.v2);
+  float g1 = fmaxf(b2 - b1, 0.0f);
+  b1 = fmaxf(s, t.v2);
+  b2 = fminf(e, t.v3);
+  float g2 = (b2 > b1) ? (b2 - b1) * (2.0f * t.v3 - b1 - b2) / (2.0f * (t.v3 - t.v2)) : 0.0f;
+  return g0 + g1 + g2;
+}
+
+// ---------------------------------------------------------------------------
+// Y[pl][c][j] = sum_iz xT[c][iz] * cfac * gamma_j.  Block: 4 waves, one
+// column each; grid (ceil(ncol / 4), views in the chunk).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, const float* __restrict__ xT,
+                                                        float* __restrict__ Y) {
+  __shared__ float lds[4][5][kMaxNz];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ncol = a.g.nx * a.g.ny;
+  const int c = blockIdx.x * 4 + wv, pl = blockIdx.y;
+  const bool live = c < ncol;
+  if (live) {
+    const int ix = c % a.g.nx, iy = c / a.g.nx;
+    const ColumnAxial ca = column_axial(a, a.g.views[p0 + pl], ix, iy);
+    for (int iz = lane; iz < a.nz; iz += 64) {
+      const Trap t = voxel_trap(a, ca, iz);
+      lds[wv][0][iz] = t.v0;
+      lds[wv][1][iz] = t.v1;
+      lds[wv][2][iz] = t.v2;
+      lds[wv][3][iz] = t.v3;
+      lds[wv][4][iz] = xT[(size_t)c * a.nz + iz] * t.cf;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  float* yrow = Y + ((size_t)pl * ncol + c) * a.nv;
+  for (int j = lane; j < a.nv; j += 64) {
+    const float fj = (float)j;
+    // first voxel whose trapezoid reaches row j (v3 grows with iz)
+    int lo = 0, hi = a.nz;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (lds[wv][3][mid] >= fj) hi = mid;
+      else lo = mid + 1;
+    }
+    float acc = 0.0f;
+    for (int iz = lo; iz < a.nz && lds[wv][0][iz] < fj + 1.0f; ++iz) {
+      Trap t;
+      t.v0 = lds[wv][0][iz];
+      t.v1 = lds[wv][1][iz];
+      t.v2 = lds[wv][2][iz];
+      t.v3 = lds[wv][3][iz];
+      const float gm = trap_gamma(fj, t);
+      if (gm > kGammaEps) acc = __builtin_fmaf(lds[wv][4][iz], gm, acc);
+    }
+    yrow[j] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// proj[p][j][u] = sum over the columns c of bin u's strip of conv(u, c) Y[c][j].
+// Wave per (view, bin); the strip walk is the 2-D B1 A kernel's with the bin
+// wave-uniform.  Grid: ceil(nu / 4) * nview blocks, view = blockIdx % nview
+// (consecutive blocks, which the dispatcher spreads over the XCDs, take
+// different views; the blocks of one view share an XCD's L2 when nview is 8).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int nview, const float* __restrict__ Y,
+                                                       float* __restrict__ proj) {
+  const Proj2DArgs& g = a.g;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int pl = blockIdx.x % nview;
+  const int u = (blockIdx.x / nview) * 4 + wv;
+  if (u >= g.nu) return;
+  const int v = p0 + pl;
+  const int ncol = g.nx * g.ny;
+  const ViewGeo vg = g.views[v];
+  const B1Bin rec = reinterpret_cast<const B1Bin*>(g.bins)[(size_t)v * g.nu + u];
+
+  const double eax = vg.e0x + u * vg.pux, eay = vg.e0y + u * vg.puy;
+  const double ebx = eax + vg.pux, eby = eay + vg.puy;
+  const double mx = eax + 0.5 * vg.pux, my = eay + 0.5 * vg.puy;
+  const bool rows = fabs(my) >= fabs(mx);
+  const double acx = -eay * g.dx, acy = eax * g.dy, ac0 = eax * vg.w0y - eay * vg.w0x;
+  const double bcx = -eby * g.dx, bcy = ebx * g.dy, bc0 = ebx * vg.w0y - eby * vg.w0x;
+  // clamped end bins (A_B1_3D.cu:150-151, as in the 2-D kernel)
+  const double lax = (u == 0) ? eax - vg.pux : eax, lay = (u == 0) ? eay - vg.puy : eay;
+  const double lbx = (u == g.nu - 1) ? ebx + vg.pux : ebx, lby = (u == g.nu - 1) ? eby + vg.puy : eby;
+  const double qcx = -lay * g.dx, qcy = lax * g.dy, qc0 = lax * vg.w0y - lay * vg.w0x;
+  const double rcx = -lby * g.dx, rcy = lbx * g.dy, rc0 = lbx * vg.w0y - lby * vg.w0x;
+  const double ra = g.h * (fabs(qcx) + fabs(qcy)), rb = g.h * (fabs(rcx) + fabs(rcy));
+  const double sa = rows ? acx : acy, ma = rows ? acy : acx;
+  const double sb = rows ? bcx : bcy, mb = rows ? bcy : bcx;
+  const double sq = rows ? qcx : qcy, mq = rows ? qcy : qcx;
+  const double sr = rows ? rcx : rcy, mr = rows ? rcy : rcx;
+  const int nmin = rows ? g.nx : g.ny, nmaj = rows ? g.ny : g.nx;
+  const int smin = rows ? 1 : g.nx, smaj = rows ? g.nx : 1;
+  const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
+
+  int m_lo = 0, m_hi = nmaj - 1;
+  if (mq != 0.0 && mr != 0.0) {
+    const double last = nmin - 1;
+    double c1 = (-ra - qc0) / mq, c2 = (-ra - qc0 - sq * last) / mq;
+    double c3 = (rb - rc0) / mr, c4 = (rb - rc0 - sr * last) / mr;
+    double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
+    m_lo = (int)fmax(floor(lo) - 1.0, 0.0);
+    m_hi = (int)fmin(ceil(hi) + 1.0, (double)(nmaj - 1));
+  }
+  double lo0 = 0.0, lo1 = 0.0, hi0 = nmin - 1, hi1 = 0.0;
+  if (sq > 0.0 && sr > 0.0) {
+    lo0 = (-ra - qc0) / sq; lo1 = -mq / sq; hi0 = (rb - rc0) / sr; hi1 = -mr / sr;
+  } else if (sq < 0.0 && sr < 0.0) {
+    hi0 = (-ra - qc0) / sq; hi1 = -mq / sq; lo0 = (rb - rc0) / sr; lo1 = -mr / sr;
+  }
+  const double an = (double)rec.ka * sa, am = (double)rec.ka * ma, a0 = (double)rec.ka * ac0;
+  const double bn = (double)rec.kb * sb, bm = (double)rec.kb * mb, b0 = (double)rec.kb * bc0;
+
+  double acc[kMaxRowChunks];
+#pragma unroll
+  for (int k = 0; k < kMaxRowChunks; ++k) acc[k] = 0.0;
+  const float* ybase = Y + (size_t)pl * ncol * a.nv + lane;
+  for (int m = m_lo; m <= m_hi; ++m) {
+    const int n0 = (int)fmax(ceil(fma((double)m, lo1, lo0)), 0.0);
+    const int n1 = (int)fmin(floor(fma((double)m, hi1, hi0)), (double)(nmin - 1));
+    if (n0 > n1) continue;
+    float ta = (float)fma((double)n0, an, fma((double)m, am, a0));
+    float tb = (float)fma((double)n0, bn, fma((double)m, bm, b0));
+    float accl[kMaxRowChunks];
+#pragma unroll
+    for (int k = 0; k < kMaxRowChunks; ++k) accl[k] = 0.0f;
+    for (int n = n0; n <= n1; ++n) {
+      const float w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
+                                s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+      const float* yc = ybase + (size_t)(m * smaj + n * smin) * a.nv;
+#pragma unroll
+      for (int k = 0; k < kMaxRowChunks; ++k)
+        if (k < a.nrc && k * 64 + lane < a.nv) accl[k] = __builtin_fmaf(w, yc[k * 64], accl[k]);
+      ta += dta;
+      tb += dtb;
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxRowChunks; ++k) acc[k] += (double)accl[k];
+  }
+  float* out = proj + (size_t)v * a.nv * g.nu + u;
+#pragma unroll
+  for (int k = 0; k < kMaxRowChunks; ++k) {
+    const int j = k * 64 + lane;
+    if (k < a.nrc && j < a.nv) out[(size_t)j * g.nu] = (float)acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Yb[pl][c][j] = sum_u conv(u, c) projT[pl][u][j].  Wave per (column, view),
+// the column's clamped corner range (A_B1_3D.cu:117-151) wave-uniform.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, const float* __restrict__ projT,
+                                                       float* __restrict__ Yb) {
+  const Proj2DArgs& g = a.g;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ncol = g.nx * g.ny;
+  const int c = blockIdx.x * 4 + wv, pl = blockIdx.y;
+  if (c >= ncol) return;
+  const int v = p0 + pl;
+  const int ix = c % g.nx, iy = c / g.nx;
+  const ViewGeo& vg = g.views[v];
+  const B1Bin* bins = reinterpret_cast<const B1Bin*>(g.bins) + (size_t)v * g.nu;
+  const double wx = ix * g.dx + vg.w0x, wy = iy * g.dy + vg.w0y;
+  const double n0 = vg.e0x * wy - vg.e0y * wx;
+  const double be = vg.pux * wy - vg.puy * wx;
+  const float bef = (float)be;
+  const float kc = -(float)n0 * __builtin_amdgcn_rcpf(bef);
+  const float gx = __builtin_fmaf(kc, vg.gx1, vg.gx0), gy = __builtin_fmaf(kc, vg.gy1, vg.gy0);
+  const float hw = g.h * (__builtin_fabsf(gx) + __builtin_fabsf(gy)) * __builtin_amdgcn_rcpf(__builtin_fabsf(bef) - vg.db);
+  const float last = (float)(g.nu - 1);
+  const int ulo = (int)fminf(fmaxf(floorf(kc - hw), 0.0f), last);
+  const int uhi = (int)fminf(fmaxf(floorf(kc + hw), 0.0f), last);
+  float acc[kMaxRowChunks];
+#pragma unroll
+  for (int k = 0; k < kMaxRowChunks; ++k) acc[k] = 0.0f;
+  float na = (float)fma((double)ulo, be, n0);
+  for (int u = ulo; u <= uhi; ++u) {
+    const B1Bin rec = bins[u];
+    const float nb = na + bef;
+    const float ta = rec.ka * na, tb = rec.kb * nb;
+    const float w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
+                              s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+    const float* row = projT + ((size_t)pl * g.nu + u) * a.nv + lane;
+#pragma unroll
+    for (int k = 0; k < kMaxRowChunks; ++k)
+      if (k < a.nrc && k * 64 + lane < a.nv) acc[k] = __builtin_fmaf(w, row[k * 64], acc[k]);
+    na = nb;
+  }
+  float* out = Yb + ((size_t)pl * ncol + c) * a.nv + lane;
+#pragma unroll
+  for (int k = 0; k < kMaxRowChunks; ++k)
+    if (k < a.nrc && k * 64 + lane < a.nv) out[k * 64] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// xT[c][iz] (+)= sum over the chunk's views of cfac * sum_j gamma_j Yb[c][j].
+// Wave per column, lanes over voxels; grid ceil(ncol / 4).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, int nview, const float* __restrict__ Yb,
+                                                        float* __restrict__ xT, int accumulate) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ncol = a.g.nx * a.g.ny;
+  const int c = blockIdx.x * 4 + wv;
+  if (c >= ncol) return;
+  const int ix = c % a.g.nx, iy = c / a.g.nx;
+  constexpr int kSlots = kMaxNz / 64;
+  double acc[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) acc[s] = 0.0;
+  const float lastj = (float)(a.nv - 1);
+  for (int pl = 0; pl < nview; ++pl) {
+    const ColumnAxial ca = column_axial(a, a.g.views[p0 + pl], ix, iy);
+    const float* yrow = Yb + ((size_t)pl * ncol + c) * a.nv;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      const int iz = s * 64 + lane;
+      if (iz >= a.nz) continue;
+      const Trap t = voxel_trap(a, ca, iz);
+      const int jlo = (int)fminf(fmaxf(floorf(t.v0), 0.0f), lastj);
+      const int jhi = (int)fminf(fmaxf(floorf(t.v3), 0.0f), lastj);
+      float sum = 0.0f;
+      for (int j = jlo; j <= jhi; ++j) {
+        const float gm = trap_gamma((float)j, t);
+        if (gm > kGammaEps) sum = __builtin_fmaf(gm, yrow[j], sum);
+      }
+      acc[s] += (double)(t.cf * sum);
+    }
+  }
+  float* out = xT + (size_t)c * a.nz;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const int iz = s * 64 + lane;
+    if (iz < a.nz) out[iz] = (float)((accumulate ? (double)out[iz] : 0.0) + acc[s]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// out[b][c][r] = in[b][r][c] for a batch of rows x cols matrices.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                        int rows, int cols) {
+  __shared__ float tile[32][33];
+  const size_t off = (size_t)blockIdx.z * rows * cols;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    if (r < rows && c < cols) tile[k][tx] = in[off + (size_t)r * cols + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + tx;
+    if (r < rows && c < cols) out[off + (size_t)c * rows + r] = tile[tx][k];
+  }
+}
+
+hipError_t launch_transpose(const float* in, float* out, int rows, int cols, int batch, hipStream_t s) {
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32, batch);
+  transpose_kernel<<<grid, dim3(256), 0, s>>>(in, out, rows, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_b1_3d_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y,
+                                      float* proj, hipStream_t s) {
+  const int ncol = a.g.nx * a.g.ny;
+  axial_fwd_kernel<<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, xT, Y);
+  conv_fwd_kernel<<<dim3(((a.g.nu + 3) / 4) * nview), dim3(256), 0, s>>>(a, p0, nview, Y, proj);
+  return hipGetLastError();
+}
+
+hipError_t launch_b1_3d_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb,
+                                       float* xT, int accumulate, hipStream_t s) {
+  const int ncol = a.g.nx * a.g.ny;
+  conv_bwd_kernel<<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, projT, Yb);
+  axial_bwd_kernel<<<dim3((ncol + 3) / 4), dim3(256), 0, s>>>(a, p0, nview, Yb, xT, accumulate);
+  return hipGetLastError();
+}
+
+}  // namespace sgdct

@@ -36,6 +36,12 @@ hipError_t launch_resid(const float* b, const float* ax, const float* w, float* 
 hipError_t launch_update(float* x, const float* c, const float* v, float lam, int positivity, size_t n, size_t per,
                          hipStream_t s);
 hipError_t launch_axpy(float* x, const float* g, float a, size_t n, hipStream_t s);
+hipError_t launch_transpose(const float* in, float* out, int rows, int cols, int batch, hipStream_t s);
+hipError_t launch_b1_3d_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y,
+                                      float* proj, hipStream_t s);
+hipError_t launch_b1_3d_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb,
+                                       float* xT, int accumulate, hipStream_t s);
+constexpr int kMaxNz3D = 512, kMaxNv3D = 512;
 }  // namespace sgdct
 
 using namespace sgdct;
@@ -92,6 +98,13 @@ struct sgdct_projector {
   long long evals = -1;
   bool bwd_lds = false;  // A^T stages bin windows in LDS (B0 / B1)
   Proj2DArgs args{};
+  // cone beam (BASIS_B1_3D): volume nz slices, detector nv rows
+  int nz = 1, nv = 1;
+  double dz = 0, dv = 0;
+  int chunk = 0;              // views per pass of the staged kernels
+  float* d_xT = nullptr;      // [ncol][nz] column-major volume
+  float* d_Y = nullptr;       // [chunk][ncol][nv] per-column row profiles
+  float* d_projT = nullptr;   // [chunk][nu][nv]
 
   ~sgdct_projector() {
     if (d_views) (void)hipFree(d_views);
@@ -99,6 +112,9 @@ struct sgdct_projector {
     if (d_img) (void)hipFree(d_img);
     if (d_sino) (void)hipFree(d_sino);
     if (d_count) (void)hipFree(d_count);
+    if (d_xT) (void)hipFree(d_xT);
+    if (d_Y) (void)hipFree(d_Y);
+    if (d_projT) (void)hipFree(d_projT);
     if (h_img) (void)hipHostFree(h_img);
     if (h_sino) (void)hipHostFree(h_sino);
   }
@@ -229,9 +245,12 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   return h;
 }
 
+size_t img_elems(const sgdct_projector* h) { return (size_t)h->nx * h->ny * h->nz; }
+size_t sino_elems(const sgdct_projector* h) { return (size_t)h->np * h->nu * h->nv; }
+
 int ensure_staging(sgdct_projector* h) {
   if (h->d_img) return SGDCT_OK;
-  const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu;
+  const size_t ni = img_elems(h), ns = sino_elems(h);
   hipError_t e = hipMalloc(&h->d_img, ni * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&h->d_sino, ns * sizeof(float));
   if (e == hipSuccess) e = hipHostMalloc(&h->h_img, ni * sizeof(float));
@@ -240,7 +259,74 @@ int ensure_staging(sgdct_projector* h) {
   return SGDCT_OK;
 }
 
+Cone3DArgs cone_args(const sgdct_projector* h) {
+  Cone3DArgs a;
+  a.g = h->args;
+  a.nz = h->nz;
+  a.nv = h->nv;
+  a.nrc = (h->nv + 63) / 64;
+  a.vc = (float)(0.5 * h->nv);
+  a.ldz = (float)(h->lsd * h->dz / h->dv);
+  a.cvol = (float)(h->dx * h->dy * h->dz);
+  a.dz = (float)h->dz;
+  a.du = h->du;
+  return a;
+}
+
+int ensure_cone_workspace(sgdct_projector* h) {
+  if (h->d_xT) return SGDCT_OK;
+  const size_t ncol = (size_t)h->nx * h->ny;
+  const size_t per_view = ncol * h->nv * sizeof(float);
+  size_t chunk = std::max<size_t>(1, ((size_t)1 << 30) / per_view);
+  if (chunk >= 8) chunk -= chunk % 8;
+  h->chunk = (int)std::min<size_t>(chunk, (size_t)h->np);
+  hipError_t e = hipMalloc(&h->d_xT, ncol * h->nz * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&h->d_Y, (size_t)h->chunk * per_view);
+  if (e == hipSuccess) e = hipMalloc(&h->d_projT, (size_t)h->chunk * h->nu * h->nv * sizeof(float));
+  if (e != hipSuccess) return hip_fail(e, "cone-beam workspace");
+  return SGDCT_OK;
+}
+
+// Volume [nz][ny][nx] -> projections [np][nv][u], `batch` back to back.
+int cone_forward(sgdct_projector* h, const float* vol, float* proj, int batch, hipStream_t s) {
+  int rc = ensure_cone_workspace(h);
+  if (rc) return rc;
+  const Cone3DArgs a = cone_args(h);
+  const int ncol = h->nx * h->ny;
+  for (int b = 0; b < batch; ++b) {
+    const float* x = vol + (size_t)b * ncol * h->nz;
+    float* y = proj + (size_t)b * h->np * h->nv * h->nu;
+    hipError_t e = launch_transpose(x, h->d_xT, h->nz, ncol, 1, s);
+    for (int p0 = 0; p0 < h->np && e == hipSuccess; p0 += h->chunk)
+      e = launch_b1_3d_forward_chunk(a, p0, std::min(h->chunk, h->np - p0), h->d_xT, h->d_Y, y, s);
+    if (e != hipSuccess) return hip_fail(e, "cone-beam forward launch");
+  }
+  return SGDCT_OK;
+}
+
+int cone_backward(sgdct_projector* h, const float* proj, float* vol, int batch, hipStream_t s) {
+  int rc = ensure_cone_workspace(h);
+  if (rc) return rc;
+  const Cone3DArgs a = cone_args(h);
+  const int ncol = h->nx * h->ny;
+  for (int b = 0; b < batch; ++b) {
+    const float* y = proj + (size_t)b * h->np * h->nv * h->nu;
+    float* x = vol + (size_t)b * ncol * h->nz;
+    hipError_t e = hipSuccess;
+    for (int p0 = 0; p0 < h->np && e == hipSuccess; p0 += h->chunk) {
+      const int nview = std::min(h->chunk, h->np - p0);
+      e = launch_transpose(y + (size_t)p0 * h->nv * h->nu, h->d_projT, h->nv, h->nu, nview, s);
+      if (e == hipSuccess)
+        e = launch_b1_3d_backward_chunk(a, p0, nview, h->d_projT, h->d_Y, h->d_xT, p0 > 0, s);
+    }
+    if (e == hipSuccess) e = launch_transpose(h->d_xT, x, ncol, h->nz, 1, s);
+    if (e != hipSuccess) return hip_fail(e, "cone-beam backward launch");
+  }
+  return SGDCT_OK;
+}
+
 int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
+  if (h->basis == BASIS_B1_3D) return cone_forward(h, img, sino, batch, s);
   hipError_t e;
   if (is_hex(h->basis)) e = launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, s);
   else if (h->basis == BASIS_SF) e = launch_sf_forward(h->args, img, sino, batch, s);
@@ -249,6 +335,7 @@ int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hi
 }
 
 int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
+  if (h->basis == BASIS_B1_3D) return cone_backward(h, sino, img, batch, s);
   hipError_t e;
   if (is_hex(h->basis)) e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
   else if (h->basis == BASIS_SF) e = launch_sf_backward(h->args, sino, img, batch, s);
@@ -256,8 +343,49 @@ int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, h
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "backward launch");
 }
 
+// Footprint evaluations of the reference's cone-beam forward
+// (A_B1_3D.cu:261 runs once per bin of the column's clamped corner range for
+// every slice before the first whose axial trapezoid starts past the last
+// row, A_B1_3D.cu:204).  Host FP64, from the handle's geometry.
+long long cone_eval_count(const sgdct_projector* h, const std::vector<ViewGeo>& views) {
+  long long total = 0;
+  const double vc = 0.5 * h->nv, ldz = h->lsd * h->dz / h->dv, zoff = 0.5 * (h->nz - 1);
+  for (int p = 0; p < h->np; ++p) {
+    const ViewGeo& v = views[p];
+    for (int iy = 0; iy < h->ny; ++iy)
+      for (int ix = 0; ix < h->nx; ++ix) {
+        double umin = 1e300, umax = -1e300;
+        for (int k = 0; k < 4; ++k) {
+          const double wx = (ix + ((k & 1) ? 1.0 : -1.0)) * h->dx + v.w0x;
+          const double wy = (iy + ((k & 2) ? 1.0 : -1.0)) * h->dy + v.w0y;
+          const double uu = (v.e0x * wy - v.e0y * wx) / std::fabs(v.pux * wy - v.puy * wx);
+          umin = std::min(umin, uu);
+          umax = std::max(umax, uu);
+        }
+        const int lo = std::min(std::max(0, (int)std::floor(umin)), h->nu - 1);
+        const int hi = std::min(std::max(0, (int)std::floor(umax)), h->nu - 1);
+        const double wy = iy * h->dy + v.w0y;
+        const double da = -(v.pux * wy - v.puy * ((ix - 1) * h->dx + v.w0x)) / h->du;
+        const double db = -(v.pux * wy - v.puy * ((ix + 1) * h->dx + v.w0x)) / h->du;
+        const double kmin = std::min(ldz / da, ldz / db);
+        // first slice with vc + kmin (iz - zoff - 1/2) >= nv
+        const double first = std::ceil((h->nv - vc) / kmin + zoff + 0.5);
+        const long long nzeff = (long long)std::min<double>(std::max(first, 0.0), (double)h->nz);
+        total += (long long)(hi - lo + 1) * nzeff;
+      }
+  }
+  return total;
+}
+
 long long eval_count(sgdct_projector* h) {
   if (h->evals >= 0) return h->evals;
+  if (h->basis == BASIS_B1_3D) {
+    std::vector<ViewGeo> views(h->np);
+    hipError_t e = hipMemcpy(views.data(), h->d_views, sizeof(ViewGeo) * h->np, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "evaluation count");
+    h->evals = cone_eval_count(h, views);
+    return h->evals;
+  }
   hipError_t e = hipSuccess;
   if (!h->d_count) e = hipMalloc(&h->d_count, sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMemset(h->d_count, 0, sizeof(unsigned long long));
@@ -273,6 +401,7 @@ long long eval_count(sgdct_projector* h) {
 // Host-array forward: x[ix*ny+iy] (FP64) -> b[p*nu+u] (FP64).
 int host_forward(double* b, double* x, sgdct_projector* h) {
   if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
+  if (h->basis == BASIS_B1_3D) return fail(SGDCT_EINVAL, "cone-beam handle: use b1_3d_forward_projection");
   try {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "set device");
@@ -300,6 +429,7 @@ int host_forward(double* b, double* x, sgdct_projector* h) {
 // Host-array back-projection: b[p*nu+u] -> x[ix*ny+iy] (overwritten).
 int host_backward(double* b, double* x, sgdct_projector* h) {
   if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
+  if (h->basis == BASIS_B1_3D) return fail(SGDCT_EINVAL, "cone-beam handle: use b1_3d_backward_projection");
   try {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "set device");
@@ -316,6 +446,54 @@ int host_backward(double* b, double* x, sgdct_projector* h) {
     if (e != hipSuccess) return hip_fail(e, "download image");
     for (int ix = 0; ix < nx; ++ix)
       for (int iy = 0; iy < ny; ++iy) x[(size_t)ix * ny + iy] = (double)h->h_img[(size_t)iy * nx + ix];
+    return SGDCT_OK;
+  } catch (...) {
+    return fail(SGDCT_EDEVICE, "unexpected exception in backward");
+  }
+}
+
+// Cone-beam host arrays: x copied raw ([iz][iy][ix], A_B1_3D.cu:716),
+// b = [p][v][u] (A_B1_3D.cu:731).
+long long cone_host_forward(double* b, double* x, sgdct_projector* h) {
+  if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
+  if (h->basis != BASIS_B1_3D) return fail(SGDCT_EINVAL, "not a cone-beam handle");
+  try {
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "set device");
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    const size_t ni = img_elems(h), ns = sino_elems(h);
+    for (size_t i = 0; i < ni; ++i) h->h_img[i] = (float)x[i];
+    e = hipMemcpy(h->d_img, h->h_img, ni * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "upload volume");
+    rc = run_forward(h, h->d_img, h->d_sino, 1, nullptr);
+    if (rc) return rc;
+    e = hipMemcpy(h->h_sino, h->d_sino, ns * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "download projections");
+    for (size_t i = 0; i < ns; ++i) b[i] = (double)h->h_sino[i];
+    return eval_count(h);
+  } catch (...) {
+    return fail(SGDCT_EDEVICE, "unexpected exception in forward");
+  }
+}
+
+int cone_host_backward(double* b, double* x, sgdct_projector* h) {
+  if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
+  if (h->basis != BASIS_B1_3D) return fail(SGDCT_EINVAL, "not a cone-beam handle");
+  try {
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "set device");
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    const size_t ni = img_elems(h), ns = sino_elems(h);
+    for (size_t i = 0; i < ns; ++i) h->h_sino[i] = (float)b[i];
+    e = hipMemcpy(h->d_sino, h->h_sino, ns * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "upload projections");
+    rc = run_backward(h, h->d_sino, h->d_img, 1, nullptr);
+    if (rc) return rc;
+    e = hipMemcpy(h->h_img, h->d_img, ni * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "download volume");
+    for (size_t i = 0; i < ni; ++i) x[i] = (double)h->h_img[i];
     return SGDCT_OK;
   } catch (...) {
     return fail(SGDCT_EDEVICE, "unexpected exception in backward");
@@ -374,6 +552,39 @@ sgdct_projector* hl_init(int nl, int nc, int np, int nu, double dt, double du, d
 }
 int hl_forward_projection(double* b, double* x, sgdct_projector* h) { return host_forward(b, x, h); }
 int hl_backward_projection(double* b, double* x, sgdct_projector* h) { return host_backward(b, x, h); }
+
+// layers/A_B1_3D.hh:24-26 / A_B1_3D.cu:701-770.
+sgdct_projector* b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, double dx, double dy, double dz,
+                            double du, double dv, double lsd, double lso, const double* angles) {
+  if (nz <= 0 || nv <= 0 || !(dz > 0) || !(dv > 0)) {
+    fail(SGDCT_EINVAL, "invalid cone-beam geometry arguments");
+    return nullptr;
+  }
+  if (nz > kMaxNz3D || nv > kMaxNv3D) {
+    fail(SGDCT_EUNSUP, "cone-beam projector supports nz <= 512 and nv <= 512");
+    return nullptr;
+  }
+  sgdct_projector* h = create(BASIS_B1, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, nullptr);
+  if (!h) return nullptr;
+  h->basis = BASIS_B1_3D;
+  h->nz = nz;
+  h->nv = nv;
+  h->dz = dz;
+  h->dv = dv;
+  return h;
+}
+
+unsigned long long b1_3d_forward_projection(double* b, double* x, sgdct_projector* h) {
+  return (unsigned long long)cone_host_forward(b, x, h);
+}
+
+int b1_3d_backward_projection(double* b, double* x, sgdct_projector* h) { return cone_host_backward(b, x, h); }
+
+int sgdct_shape3(const sgdct_projector* h, int* out) {
+  if (!h || !out) return fail(SGDCT_EINVAL, "null argument");
+  out[0] = h->nx; out[1] = h->ny; out[2] = h->nz; out[3] = h->np; out[4] = h->nu; out[5] = h->nv;
+  return h->basis;
+}
 
 void sgdct_destroy(sgdct_projector* h) { delete h; }
 

@@ -20,7 +20,7 @@
 
 namespace sgdct {
 
-enum Basis : int { BASIS_B0 = 0, BASIS_B1 = 1, BASIS_SF = 2, BASIS_H0 = 3, BASIS_HL = 4 };
+enum Basis : int { BASIS_B0 = 0, BASIS_B1 = 1, BASIS_SF = 2, BASIS_H0 = 3, BASIS_HL = 4, BASIS_B1_3D = 5 };
 
 // Per-view FP64 constants.  With w = X - p = (ix dx + w0x, iy dy + w0y):
 //   N_k(ix, iy) = cross(e0 + k pu, w),   beta(ix, iy) = cross(pu, w) (< 0).
@@ -42,6 +42,17 @@ struct Proj2DArgs {
   double dx, dy;
   float h;               // corner half-extent in index units (1 for B1, 1/2 for B0/SF)
   int nchunk;            // ceil(nu / 64)
+};
+
+// Cone-beam B1 (layers/A_B1_3D.cu): the in-plane part reuses Proj2DArgs.
+struct Cone3DArgs {
+  Proj2DArgs g;     // in-plane geometry and B1 bin records (h = 1)
+  int nz, nv, nrc;  // nrc = ceil(nv / 64)
+  float vc;         // detector row of the source plane (nv / 2)
+  float ldz;        // lsd * dz / dv: v = vc + ldz * (z index offset) / depth
+  float cvol;       // dx * dy * dz
+  float dz;
+  double du;        // |pu|
 };
 
 constexpr int kMaxWindow = 128;   // bins of one view cached in LDS per pixel tile

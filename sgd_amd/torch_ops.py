@@ -52,6 +52,8 @@ class Projector:
             raise _lib.SgdctError("%s_init failed: %s" % (pre, _lib.last_error()))
         self.handle = h
         self.device = torch.device("cuda", torch.cuda.current_device())
+        self._img_shape = (self.ny, self.nx)
+        self._sino_shape = (self.np, self.nu)
 
     @classmethod
     def hexagonal(cls, basis, nl, nc, angles, nu, dt, du, lsd, lso, drho=None):
@@ -61,6 +63,31 @@ class Projector:
         if basis not in ("H0", "Hl"):
             raise ValueError("hexagonal bases are H0 and Hl")
         return cls(basis, nc, nl, angles, nu, dt, dt, du, lsd, lso, drho=drho)
+
+    @classmethod
+    def cone_beam(cls, nx, ny, nz, angles, nu, nv, dx, dy, dz, du, dv, lsd, lso):
+        """Cone-beam B1 (b1_3d_init): volumes (nz, ny, nx), projections
+        (np, nv, nu); A^T is the transpose of A."""
+        if not torch.cuda.is_available():
+            raise _lib.SgdctError("sgd_amd projectors need a ROCm GPU (torch.cuda unavailable)")
+        self = cls.__new__(cls)
+        self.basis = "B1_3D"
+        self.nx, self.ny, self.nz, self.nu, self.nv = int(nx), int(ny), int(nz), int(nu), int(nv)
+        ang = np.ascontiguousarray(angles, dtype=np.float64)
+        self.np = int(ang.size)
+        self.angles = ang
+        self.geom = dict(dx=float(dx), dy=float(dy), dz=float(dz), du=float(du), dv=float(dv), lsd=float(lsd),
+                         lso=float(lso))
+        torch.cuda.init()
+        h = _lib.lib().b1_3d_init(self.nx, self.ny, self.nz, self.np, self.nu, self.nv, dx, dy, dz, du, dv, lsd, lso,
+                                  _lib.dptr(ang))
+        if not h:
+            raise _lib.SgdctError("b1_3d_init failed: %s" % _lib.last_error())
+        self.handle = h
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self._img_shape = (self.nz, self.ny, self.nx)
+        self._sino_shape = (self.np, self.nv, self.nu)
+        return self
 
     def close(self):
         if getattr(self, "handle", None):
@@ -75,11 +102,11 @@ class Projector:
 
     @property
     def img_shape(self):
-        return (self.ny, self.nx)
+        return self._img_shape
 
     @property
     def sino_shape(self):
-        return (self.np, self.nu)
+        return self._sino_shape
 
     def eval_count(self):
         return _lib.check(_lib.lib().sgdct_eval_count(self.handle), "eval_count")
@@ -87,13 +114,14 @@ class Projector:
     def _check_in(self, t, tail):
         if not (t.is_cuda and t.dtype == torch.float32):
             raise TypeError("expected a float32 CUDA tensor")
-        if tuple(t.shape[-2:]) != tail:
-            raise ValueError("trailing shape %s, expected %s" % (tuple(t.shape[-2:]), tail))
+        k = len(tail)
+        if tuple(t.shape[-k:]) != tail:
+            raise ValueError("trailing shape %s, expected %s" % (tuple(t.shape[-k:]), tail))
         return t.contiguous()
 
     def forward(self, img, out=None):
         img = self._check_in(img, self.img_shape)
-        lead = img.shape[:-2]
+        lead = img.shape[:-len(self.img_shape)]
         batch = int(np.prod(lead)) if len(lead) else 1
         if out is None:
             out = torch.empty(lead + self.sino_shape, device=img.device, dtype=torch.float32)
@@ -105,7 +133,7 @@ class Projector:
 
     def backward(self, sino, out=None):
         sino = self._check_in(sino, self.sino_shape)
-        lead = sino.shape[:-2]
+        lead = sino.shape[:-len(self.sino_shape)]
         batch = int(np.prod(lead)) if len(lead) else 1
         if out is None:
             out = torch.empty(lead + self.img_shape, device=sino.device, dtype=torch.float32)
