@@ -72,6 +72,22 @@ sgdct_projector *hl_init(int nl, int nc, int np, int nu, double dt, double du, d
 int hl_forward_projection(double *b, double *x, sgdct_projector *h);
 int hl_backward_projection(double *b, double *x, sgdct_projector *h);
 
+/* layers/A_B0.cu:449-492 (pixel_matrix_generation) and A_B1.cu:545-593
+ * (bilinear_matrix_generation): the explicit system matrix of B0 / B1 for np
+ * equally spaced views over [0, 2 pi (np-1)/np], as COO triplets
+ * spmat[3k..3k+2] = (p*nu + u, i*ny + j, conv) in the order view, bin, i, j,
+ * kept where conv > 0 (no dx*dy factor, as the reference).  Returns the
+ * triplet count, or a negative code (also when more than buffer_size triplets
+ * are needed; the reference asserts).  B1 rows are the footprint of
+ * b1_forward_projection: the reference's matrix kernel averages over
+ * [y, y + vblur] (A_B1.cu:402-411), which returns bin u-1's value in row u;
+ * its projector and its quadrature generator python/matrix_gen/
+ * A_B1_ref_project.py agree with the rows returned here. */
+int pixel_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
+                            double lso, double *spmat, int buffer_size);
+int bilinear_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
+                               double lso, double *spmat, int buffer_size);
+
 /* layers/A_B1_3D.hh:24-26 / A_B1_3D.cu:701 (b1_3d_init), :709 (forward),
  * :744 (backward).  Cone beam, in-plane B1 footprint times an axial
  * trapezoid; volume x copied raw as [iz][iy][ix] (A_B1_3D.cu:716),
@@ -125,6 +141,30 @@ int sgdct_sirt_residual(const float *b, const float *ax, const float *w, float *
 int sgdct_sirt_update(float *x, const float *c, const float *v, float lam, int positivity, long long n,
                       long long per, void *stream);
 int sgdct_axpy(float *x, const float *g, float a, long long n, void *stream);
+
+/* ---- batched slices (no reference counterpart: the reference projects one
+ * slice per call and launches one kernel per view, layers/A_B1.cu:512-541) --
+ * Many B0 / B1 / SF projectors of uneven size, view count and basis run as
+ * one persistent launch per direction over a device work queue (A: 64-bin
+ * waves of one view, A^T: 16x16 pixel tiles), slices ordered by decreasing
+ * per-item cost.  Results are bit-identical to the per-projector entry
+ * points.  Images / sinograms of all slices are packed back to back in one
+ * device buffer each, at the element offsets sgdct_batch_layout returns
+ * (nslice + 1 entries each).  The projectors must outlive the batch; calls on
+ * one batch must not overlap (they share the queue counters). */
+typedef struct sgdct_batch sgdct_batch;
+sgdct_batch *sgdct_batch_create(sgdct_projector *const *slices, int nslice);
+void sgdct_batch_destroy(sgdct_batch *b);
+int sgdct_batch_layout(const sgdct_batch *b, long long *img_off, long long *sino_off);
+/* Launch shape: persistent blocks per CU for A and A^T (1..32) and slice
+ * order (0: decreasing per-item cost, 1: as given); set by the autotuner. */
+int sgdct_batch_config(sgdct_batch *b, int fwd_blocks_per_cu, int bwd_blocks_per_cu, int order);
+int sgdct_batch_forward(sgdct_batch *b, const float *img, float *sino, void *stream);
+int sgdct_batch_backward(sgdct_batch *b, const float *sino, float *img, void *stream);
+/* Same as forward (direction 0) / backward (1), through a hipGraph captured on
+ * `stream` (non-default) at the first call for these buffers and replayed
+ * afterwards; the graph clears its queue counters with a kernel, not a memset. */
+int sgdct_batch_graph_launch(sgdct_batch *b, int direction, const float *in, float *out, void *stream);
 
 /* Human-readable description of the calling thread's last error. */
 const char *sgdct_last_error(void);

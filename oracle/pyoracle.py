@@ -262,3 +262,20 @@ class Geometry3:
 
     def cell_exact(self, p, u, j, ix, iy, iz, panels=8):
         return self.lib.or_b1_3d_cell_exact(self.h, p, u, j, ix, iy, iz, panels)
+
+
+def matrix_generation(oracle, basis, np_, nu, nx, ny, dx, dy, du, lsd, lso, buffer_size=None, fixed=True):
+    """COO triplets (rows, cols, vals) of pixel_matrix_generation (B0) /
+    bilinear_matrix_generation (B1); `fixed` selects the B1 projector's
+    interval instead of the reference kernel's one-bin-off one; raises if
+    buffer_size is too small."""
+    lib = oracle.lib
+    f = lib.or_matrix_generation
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double] * 5 + [_dp, ctypes.c_longlong, ctypes.c_int]
+    cap = buffer_size or np_ * nu * nx * ny
+    buf = np.zeros((cap, 3))
+    n = f(BASES[basis], np_, nu, nx, ny, dx, dy, du, lsd, lso, _ptr(buf), cap, int(bool(fixed)))
+    if n < 0:
+        raise ValueError("buffer too small")
+    return buf[:n, 0].astype(np.int64), buf[:n, 1].astype(np.int64), buf[:n, 2].copy()

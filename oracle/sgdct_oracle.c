@@ -1342,4 +1342,103 @@ double or_b1_3d_cell_exact(const void *h, int p, int u, int j, int ix, int iy, i
     return total;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Explicit system matrices: pixel_matrix_generation (B0, layers/A_B0.cu:36-  */
+/* 121 kernel, :449-492 host) and bilinear_matrix_generation (B1,             */
+/* A_B1.cu:355-427, :545-593).  Views from geo_init_example(lsd, lso, 0,      */
+/* 2 pi (np-1)/np) (utils/geo.cu:209-230: equal steps, no 45-degree           */
+/* snapping); every (bin, pixel) pair passing the kernels' two culls with a   */
+/* positive footprint becomes a COO triplet (row p*nu + u, col i*ny + j, conv)*/
+/* in the order view, bin, i, j.  Returns the triplet count, or -1 when more  */
+/* than buffer_size triplets would be written (the reference asserts).        */
+/* B1, fixed = 0: the reference's kernel, which averages over [y, y + vblur]  */
+/* (A_B1.cu:402-411) and so returns bin u-1's footprint in row u (its own     */
+/* projector, A_B1.cu:182-197, and its quadrature generator                   */
+/* python/matrix_gen/A_B1_ref_project.py agree with each other, not with it). */
+/* fixed = 1: the projector's interval (x1 sgn, x2) -- the matrix of          */
+/* b1_forward_projection.  B0 ignores `fixed`.                                */
+/* ------------------------------------------------------------------------- */
+int or_matrix_generation(int basis, int np, int nu, int nx, int ny, double dx, double dy, double du,
+                         double lsd, double lso, double *spmat, long long buffer_size, int fixed) {
+    const double PI_ = 3.14159265358979323846;
+    double end = PI_ * 2 * (np - 1) / np;
+    double dangle = np > 1 ? end / (np - 1) : 0.0;
+    double nx2 = 0.5 * (nx - 1), ny2 = 0.5 * (ny - 1);
+    long long idx = 0;
+    double *row = (double *)malloc(sizeof(double) * (size_t)nx * ny);
+    for (int p = 0; p < np; ++p) {
+        double beta = p * dangle, c = cos(beta), sn = sin(beta);
+        double px = -lso * c, py = -lso * sn;
+        double dtx = (lsd - lso) * c, dty = (lsd - lso) * sn;
+        double pux = -du * sn, puy = du * c;
+        for (int u = 0; u < nu; ++u) {
+            double sx = dtx + pux * (u - 0.5 * nu + 0.5), sy = dty + puy * (u - 0.5 * nu + 0.5);
+            double ax = sx - pux * 0.5, ay = sy - puy * 0.5, bx = sx + pux * 0.5, by = sy + puy * 0.5;
+            double theta = atan2(fabs(px - sx), fabs(py - sy));
+            double sint = sin(theta), cost = cos(theta);
+            double vx, vy;
+            if (basis == OR_B0) {
+                vx = fabs(dx * cost);
+                vy = fabs(dy * sint);
+                if (py < sy) vx = -vx;
+                if (px < sx) vy = -vy;
+            } else {
+                vx = dx * cost;
+                vy = dy * sint;
+            }
+            for (int i = 0; i < nx; ++i)
+                for (int j = 0; j < ny; ++j) {
+                    double ox = (i - nx2) * dx, oy = (j - ny2) * dy;
+                    double num = (ox - px) * (sx - px) + (oy - py) * (sy - py);
+                    double ra = num / ((ax - px) * (sx - px) + (ay - py) * (sy - py));
+                    double rb = num / ((bx - px) * (sx - px) + (by - py) * (sy - py));
+                    double ax1 = px + (ax - px) * ra, ay1 = py + (ay - py) * ra;
+                    double bx1 = px + (bx - px) * rb, by1 = py + (by - py) * rb;
+                    row[i * ny + j] = 0.0;
+                    if (fabs(ax1 - ox) > dx && fabs(bx1 - ox) > dx && fabs(ay1 - oy) > dy && fabs(by1 - oy) > dy)
+                        continue;
+                    fpt conv;
+                    if (basis == OR_B0) {
+                        double om = -0.5 * dx * (px - sx) + 0.5 * dy * (py - sy);
+                        double l2 = (sx - px) * (sx - px) + (sy - py) * (sy - py);
+                        double zx = ox - 0.5 * dx - om * (px - sx) / l2, zy = oy + 0.5 * dy - om * (py - sy) / l2;
+                        double vblur = -sqrt((bx1 - ax1) * (bx1 - ax1) + (by1 - ay1) * (by1 - ay1));
+                        double y = sqrt((ax1 - zx) * (ax1 - zx) + (ay1 - zy) * (ay1 - zy));
+                        if (y > fabs(vx) + fabs(vy) + fabs(vblur)) continue;
+                        if ((ay - py) * (zx - px) < (zy - py) * (ax - px)) y = -y;
+                        conv = b0_conv((fpt)y, (fpt)vx, (fpt)vy, (fpt)vblur);
+                    } else {
+                        double vblur = sqrt((bx1 - ax1) * (bx1 - ax1) + (by1 - ay1) * (by1 - ay1));
+                        double y = sqrt((ax1 - ox) * (ax1 - ox) + (ay1 - oy) * (ay1 - oy));
+                        if (y > 2 * fabs(vx) + 2 * fabs(vy) + fabs(vblur)) continue;
+                        if ((ay - py) * (ox - px) > (oy - py) * (ax - px)) y = -y;
+                        fpt a = (fpt)(vx > vy ? vx : vy), b = (fpt)(vx < vy ? vx : vy);
+                        if (fixed) {
+                            double x2 = sqrt((bx1 - ox) * (bx1 - ox) + (by1 - oy) * (by1 - oy));
+                            double sg = ((ax1 - ox) * (bx1 - ox) + (ay1 - oy) * (by1 - oy) > 0) ? 1.0 : -1.0;
+                            double x1 = sqrt((ax1 - ox) * (ax1 - ox) + (ay1 - oy) * (ay1 - oy));
+                            conv = b1_conv((fpt)(x1 * sg), (fpt)x2, a, b);
+                        } else {
+                            conv = b1_conv((fpt)y, (fpt)(y + vblur), a, b);
+                        }
+                    }
+                    if (conv > 0) row[i * ny + j] = (double)conv;
+                }
+            for (int k = 0; k < nx * ny; ++k)
+                if (row[k] != 0.0) {
+                    if (idx == buffer_size) {
+                        free(row);
+                        return -1;
+                    }
+                    spmat[3 * idx] = (double)p * nu + u;
+                    spmat[3 * idx + 1] = (double)k;
+                    spmat[3 * idx + 2] = row[k];
+                    ++idx;
+                }
+        }
+    }
+    free(row);
+    return (int)idx;
+}
+
 int or_precision_bits(void) { return (int)(sizeof(fpt) * 8); }

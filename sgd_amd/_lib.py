@@ -71,6 +71,24 @@ def lib():
     L.b1_3d_forward_projection.restype = ctypes.c_ulonglong
     L.b1_3d_backward_projection.argtypes = [_dp, _dp, _h]
     L.b1_3d_backward_projection.restype = ctypes.c_int
+    for name in ("pixel_matrix_generation", "bilinear_matrix_generation"):
+        f = getattr(L, name)
+        f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [_dp, ctypes.c_int]
+        f.restype = ctypes.c_int
+    L.sgdct_batch_create.argtypes = [ctypes.POINTER(_h), ctypes.c_int]
+    L.sgdct_batch_create.restype = _h
+    L.sgdct_batch_destroy.argtypes = [_h]
+    L.sgdct_batch_destroy.restype = None
+    L.sgdct_batch_layout.argtypes = [_h, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    L.sgdct_batch_layout.restype = ctypes.c_int
+    L.sgdct_batch_config.argtypes = [_h, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.sgdct_batch_config.restype = ctypes.c_int
+    for name in ("sgdct_batch_forward", "sgdct_batch_backward"):
+        f = getattr(L, name)
+        f.argtypes = [_h, _fp, _fp, _fp]
+        f.restype = ctypes.c_int
+    L.sgdct_batch_graph_launch.argtypes = [_h, ctypes.c_int, _fp, _fp, _fp]
+    L.sgdct_batch_graph_launch.restype = ctypes.c_int
     L.sgdct_eval_count.argtypes = [_h]
     L.sgdct_eval_count.restype = ctypes.c_longlong
     L.sgdct_tv_gradient.argtypes = [_fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp]

@@ -38,7 +38,7 @@ constexpr float kGammaEps = 1e-7f; // A_B1_3D.cu:300 skips gamma <= eps
 // Axial trapezoid of voxel iz in a column: rows of the four corners
 // (x index +-1 at the column's iy, z index +-1/2; A_B1_3D.cu:133-137,
 // 188-202), sorted, and cfac (A_B1_3D.cu:209-210).
-struct Trap {
+struct AxialTrap {
   float v0, v1, v2, v3, cf;
 };
 
@@ -61,11 +61,11 @@ __device__ __forceinline__ ColumnAxial column_axial(const Cone3DArgs& a, const V
   return c;
 }
 
-__device__ __forceinline__ Trap voxel_trap(const Cone3DArgs& a, const ColumnAxial& c, int iz) {
+__device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const ColumnAxial& c, int iz) {
   const float zc = (float)iz - 0.5f * (float)(a.nz - 1);
   const float a1 = __builtin_fmaf(c.ka, zc - 0.5f, a.vc), a2 = __builtin_fmaf(c.ka, zc + 0.5f, a.vc);
   const float b1 = __builtin_fmaf(c.kb, zc - 0.5f, a.vc), b2 = __builtin_fmaf(c.kb, zc + 0.5f, a.vc);
-  Trap t;
+  AxialTrap t;
   t.v0 = fminf(a1, b1);
   t.v3 = fmaxf(a2, b2);
   const float h1 = fmaxf(a1, b1), l2 = fminf(a2, b2);
@@ -79,7 +79,7 @@ __device__ __forceinline__ Trap voxel_trap(const Cone3DArgs& a, const ColumnAxia
 
 // Overlap of [s, s+1) with the unit-height trapezoid (A_SF.cu:48-84 /
 // A_B1_3D.cu:30-66), segment by segment.
-__device__ __forceinline__ float trap_gamma(float s, const Trap& t) {
+__device__ __forceinline__ float trap_gamma(float s, const AxialTrap& t) {
   const float e = s + 1.0f;
   float b1 = fmaxf(s, t.v0), b2 = fminf(e, t.v1);
   float g0 = (b2 > b1) ? (b2 - b1) * (b2 + b1 - 2.0f * t.v0) / (2.0f * (t.v1 - t.v0)) : 0.0f;
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
     const int ix = c % a.g.nx, iy = c / a.g.nx;
     const ColumnAxial ca = column_axial(a, a.g.views[p0 + pl], ix, iy);
     for (int iz = lane; iz < a.nz; iz += 64) {
-      const Trap t = voxel_trap(a, ca, iz);
+      const AxialTrap t = voxel_trap(a, ca, iz);
       lds[wv][0][iz] = t.v0;
       lds[wv][1][iz] = t.v1;
       lds[wv][2][iz] = t.v2;
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
     }
     float acc = 0.0f;
     for (int iz = lo; iz < a.nz && lds[wv][0][iz] < fj + 1.0f; ++iz) {
-      Trap t;
+      AxialTrap t;
       t.v0 = lds[wv][0][iz];
       t.v1 = lds[wv][1][iz];
       t.v2 = lds[wv][2][iz];
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, in
     for (int s = 0; s < kSlots; ++s) {
       const int iz = s * 64 + lane;
       if (iz >= a.nz) continue;
-      const Trap t = voxel_trap(a, ca, iz);
+      const AxialTrap t = voxel_trap(a, ca, iz);
       const int jlo = (int)fminf(fmaxf(floorf(t.v0), 0.0f), lastj);
       const int jhi = (int)fminf(fmaxf(floorf(t.v3), 0.0f), lastj);
       float sum = 0.0f;

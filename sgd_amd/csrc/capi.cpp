@@ -42,6 +42,10 @@ hipError_t launch_b1_3d_forward_chunk(const Cone3DArgs& a, int p0, int nview, co
 hipError_t launch_b1_3d_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb,
                                        float* xT, int accumulate, hipStream_t s);
 constexpr int kMaxNz3D = 512, kMaxNv3D = 512;
+hipError_t launch_matgen(int basis, const Proj2DArgs& g, bool write, unsigned int* counts, const long long* offsets,
+                         double* out, hipStream_t s);
+hipError_t launch_batch_forward(const BatchPlan& P, int blocks, const float* img, float* sino, hipStream_t s);
+hipError_t launch_batch_backward(const BatchPlan& P, int blocks, const float* sino, float* img, hipStream_t s);
 }  // namespace sgdct
 
 using namespace sgdct;
@@ -117,6 +121,39 @@ struct sgdct_projector {
     if (d_projT) (void)hipFree(d_projT);
     if (h_img) (void)hipHostFree(h_img);
     if (h_sino) (void)hipHostFree(h_sino);
+  }
+};
+
+struct sgdct_batch {
+  int n = 0, device = 0, ncu = 256;
+  std::vector<long long> img_off, sino_off;  // [n + 1]
+  std::vector<SliceDesc> desc;
+  std::vector<double> fwd_cost, bwd_cost;    // per-item cost estimates
+  std::vector<long long> fwd_items, bwd_items;
+  SliceDesc* d_slices = nullptr;
+  int* d_order = nullptr;          // [2][n]
+  long long* d_prefix = nullptr;   // [2][n + 1]
+  unsigned int* d_counters = nullptr;
+  BatchPlan plan{};
+  int fwd_bpc = 8, bwd_bpc = 4, order = 0;
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    const void* in = nullptr;
+    void* out = nullptr;
+  } graph[2];
+
+  void drop_graphs() {
+    for (auto& g : graph) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = Graph{};
+    }
+  }
+  ~sgdct_batch() {
+    drop_graphs();
+    if (d_slices) (void)hipFree(d_slices);
+    if (d_order) (void)hipFree(d_order);
+    if (d_prefix) (void)hipFree(d_prefix);
+    if (d_counters) (void)hipFree(d_counters);
   }
 };
 
@@ -500,6 +537,102 @@ int cone_host_backward(double* b, double* x, sgdct_projector* h) {
   }
 }
 
+// Relative per-item costs of the batched work queue: an A item is one view's
+// 64-bin strip walk (length ~ the larger image edge), an A^T item one pixel
+// tile over all views; per-evaluation weights B1 1, SF 0.6, B0 0.45.
+double basis_weight(int basis) { return basis == BASIS_B1 ? 1.0 : (basis == BASIS_SF ? 0.6 : 0.45); }
+
+// Upload the slice order and cumulative item counts for the current `order`
+// (0: decreasing per-item cost, 1: as given).
+int batch_upload_plan(sgdct_batch* b) {
+  const int n = b->n;
+  std::vector<int> ord[2];
+  std::vector<long long> pre[2];
+  for (int dir = 0; dir < 2; ++dir) {
+    ord[dir].resize(n);
+    for (int i = 0; i < n; ++i) ord[dir][i] = i;
+    const std::vector<double>& cost = dir == 0 ? b->fwd_cost : b->bwd_cost;
+    if (b->order == 0)
+      std::stable_sort(ord[dir].begin(), ord[dir].end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    pre[dir].assign(n + 1, 0);
+    const std::vector<long long>& items = dir == 0 ? b->fwd_items : b->bwd_items;
+    for (int k = 0; k < n; ++k) pre[dir][k + 1] = pre[dir][k] + items[ord[dir][k]];
+  }
+  hipError_t e = hipMemcpy(b->d_order, ord[0].data(), sizeof(int) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_order + n, ord[1].data(), sizeof(int) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_prefix, pre[0].data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_prefix + n + 1, pre[1].data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "batch plan upload");
+  BatchPlan& P = b->plan;
+  P.slices = b->d_slices;
+  P.fwd_order = b->d_order;
+  P.bwd_order = b->d_order + n;
+  P.fwd_prefix = b->d_prefix;
+  P.bwd_prefix = b->d_prefix + n + 1;
+  P.counters = b->d_counters;
+  P.nslice = n;
+  P.nfwd = pre[0][n];
+  P.nbwd = pre[1][n];
+  return SGDCT_OK;
+}
+
+int batch_run(sgdct_batch* b, int dir, const float* in, float* out, hipStream_t s) {
+  hipError_t e = dir == 0 ? launch_batch_forward(b->plan, b->ncu * b->fwd_bpc, in, out, s)
+                          : launch_batch_backward(b->plan, b->ncu * b->bwd_bpc, in, out, s);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, dir == 0 ? "batch forward launch" : "batch backward launch");
+}
+
+// Explicit system matrix of B0 / B1 (pixel_matrix_generation,
+// bilinear_matrix_generation): views from geo_init_example(lsd, lso, 0,
+// 2 pi (np-1)/np) (utils/geo.cu:209-230), COO triplets in the reference's
+// order.  Returns the triplet count or a negative code; a buffer smaller than
+// the count is an error (the reference asserts, A_B1.cu:578-579).
+int matrix_generation(int basis, int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
+                      double lso, double* spmat, int buffer_size) {
+  if (!spmat || buffer_size < 0) return fail(SGDCT_EINVAL, "null or negative matrix buffer");
+  if (np <= 0) return fail(SGDCT_EINVAL, "np must be positive");
+  std::vector<double> angles(np);
+  const double end = M_PI * 2 * (np - 1) / np;
+  const double dangle = np > 1 ? end / (np - 1) : 0.0;
+  for (int p = 0; p < np; ++p) angles[p] = p * dangle;
+  sgdct_projector* h = create(basis, nx, ny, np, nu, dx, dy, du, lsd, lso, angles.data(), nullptr);
+  if (!h) return SGDCT_EINVAL;
+  const size_t nb = (size_t)np * nu;
+  unsigned int* d_counts = nullptr;
+  long long* d_off = nullptr;
+  double* d_out = nullptr;
+  int rc = SGDCT_OK;
+  hipError_t e = hipMalloc(&d_counts, nb * sizeof(unsigned int));
+  if (e == hipSuccess) e = launch_matgen(basis, h->args, false, d_counts, nullptr, nullptr, nullptr);
+  std::vector<unsigned int> counts(nb);
+  if (e == hipSuccess) e = hipMemcpy(counts.data(), d_counts, nb * sizeof(unsigned int), hipMemcpyDeviceToHost);
+  long long total = 0;
+  std::vector<long long> off(nb);
+  if (e == hipSuccess) {
+    for (size_t i = 0; i < nb; ++i) {
+      off[i] = total;
+      total += counts[i];
+    }
+    if (total > (long long)buffer_size) {
+      rc = fail(SGDCT_EINVAL, "matrix buffer holds " + std::to_string(buffer_size) + " triplets, " +
+                                  std::to_string(total) + " needed");
+    } else if (total > 0) {
+      e = hipMalloc(&d_off, nb * sizeof(long long));
+      if (e == hipSuccess) e = hipMemcpy(d_off, off.data(), nb * sizeof(long long), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMalloc(&d_out, (size_t)total * 3 * sizeof(double));
+      if (e == hipSuccess) e = launch_matgen(basis, h->args, true, d_counts, d_off, d_out, nullptr);
+      if (e == hipSuccess) e = hipMemcpy(spmat, d_out, (size_t)total * 3 * sizeof(double), hipMemcpyDeviceToHost);
+    }
+  }
+  if (e != hipSuccess) rc = hip_fail(e, "matrix generation");
+  if (d_counts) (void)hipFree(d_counts);
+  if (d_off) (void)hipFree(d_off);
+  if (d_out) (void)hipFree(d_out);
+  delete h;
+  return rc != SGDCT_OK ? rc : (int)total;
+}
+
 }  // namespace
 
 extern "C" {
@@ -553,6 +686,17 @@ sgdct_projector* hl_init(int nl, int nc, int np, int nu, double dt, double du, d
 int hl_forward_projection(double* b, double* x, sgdct_projector* h) { return host_forward(b, x, h); }
 int hl_backward_projection(double* b, double* x, sgdct_projector* h) { return host_backward(b, x, h); }
 
+// layers/A_B0.cu:449-492, A_B1.cu:545-593.
+int pixel_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd, double lso,
+                            double* spmat, int buffer_size) {
+  return matrix_generation(BASIS_B0, np, nu, nx, ny, dx, dy, du, lsd, lso, spmat, buffer_size);
+}
+
+int bilinear_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
+                               double lso, double* spmat, int buffer_size) {
+  return matrix_generation(BASIS_B1, np, nu, nx, ny, dx, dy, du, lsd, lso, spmat, buffer_size);
+}
+
 // layers/A_B1_3D.hh:24-26 / A_B1_3D.cu:701-770.
 sgdct_projector* b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, double dx, double dy, double dz,
                             double du, double dv, double lsd, double lso, const double* angles) {
@@ -587,6 +731,139 @@ int sgdct_shape3(const sgdct_projector* h, int* out) {
 }
 
 void sgdct_destroy(sgdct_projector* h) { delete h; }
+
+sgdct_batch* sgdct_batch_create(sgdct_projector* const* slices, int nslice) {
+  if (!slices || nslice <= 0) {
+    fail(SGDCT_EINVAL, "empty slice list");
+    return nullptr;
+  }
+  auto* b = new (std::nothrow) sgdct_batch();
+  if (!b) {
+    fail(SGDCT_ENOMEM, "host allocation failed");
+    return nullptr;
+  }
+  b->n = nslice;
+  b->img_off.assign(nslice + 1, 0);
+  b->sino_off.assign(nslice + 1, 0);
+  for (int i = 0; i < nslice; ++i) {
+    const sgdct_projector* h = slices[i];
+    if (!h || (h->basis != BASIS_B0 && h->basis != BASIS_B1 && h->basis != BASIS_SF)) {
+      fail(SGDCT_EUNSUP, "batched slices must be B0, B1 or SF projectors");
+      delete b;
+      return nullptr;
+    }
+    if (i > 0 && h->device != slices[0]->device) {
+      fail(SGDCT_EINVAL, "batched slices must live on one device");
+      delete b;
+      return nullptr;
+    }
+    SliceDesc d;
+    d.g = h->args;
+    d.basis = h->basis;
+    d.lds = h->bwd_lds ? 1 : 0;
+    d.img_off = b->img_off[i];
+    d.sino_off = b->sino_off[i];
+    b->desc.push_back(d);
+    b->img_off[i + 1] = b->img_off[i] + (long long)h->nx * h->ny;
+    b->sino_off[i + 1] = b->sino_off[i] + (long long)h->np * h->nu;
+    const double w = basis_weight(h->basis);
+    b->fwd_items.push_back((long long)h->np * h->args.nchunk);
+    b->bwd_items.push_back((long long)((h->nx + kTile - 1) / kTile) * ((h->ny + kTile - 1) / kTile));
+    b->fwd_cost.push_back(w * std::max(h->nx, h->ny));
+    b->bwd_cost.push_back(w * h->np);
+  }
+  b->device = slices[0]->device;
+  hipError_t e = hipSetDevice(b->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device);
+  if (e == hipSuccess) e = hipMalloc(&b->d_slices, sizeof(SliceDesc) * nslice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_slices, b->desc.data(), sizeof(SliceDesc) * nslice, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&b->d_order, sizeof(int) * 2 * nslice);
+  if (e == hipSuccess) e = hipMalloc(&b->d_prefix, sizeof(long long) * 2 * (nslice + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_counters, sizeof(unsigned int) * 2);
+  if (e != hipSuccess) {
+    hip_fail(e, "batch allocation");
+    delete b;
+    return nullptr;
+  }
+  if (batch_upload_plan(b) != SGDCT_OK) {
+    delete b;
+    return nullptr;
+  }
+  if (b->plan.nfwd >= (1LL << 32) || b->plan.nbwd >= (1LL << 32)) {
+    fail(SGDCT_EUNSUP, "batch has more than 2^32 work items");
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+void sgdct_batch_destroy(sgdct_batch* b) { delete b; }
+
+int sgdct_batch_layout(const sgdct_batch* b, long long* img_off, long long* sino_off) {
+  if (!b || !img_off || !sino_off) return fail(SGDCT_EINVAL, "null argument");
+  for (int i = 0; i <= b->n; ++i) {
+    img_off[i] = b->img_off[i];
+    sino_off[i] = b->sino_off[i];
+  }
+  return b->n;
+}
+
+int sgdct_batch_config(sgdct_batch* b, int fwd_blocks_per_cu, int bwd_blocks_per_cu, int order) {
+  if (!b || fwd_blocks_per_cu < 1 || fwd_blocks_per_cu > 32 || bwd_blocks_per_cu < 1 || bwd_blocks_per_cu > 32 ||
+      (order != 0 && order != 1))
+    return fail(SGDCT_EINVAL, "bad batch configuration");
+  b->drop_graphs();
+  b->fwd_bpc = fwd_blocks_per_cu;
+  b->bwd_bpc = bwd_blocks_per_cu;
+  if (order != b->order) {
+    b->order = order;
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return hip_fail(e, "set device");
+    return batch_upload_plan(b);
+  }
+  return SGDCT_OK;
+}
+
+int sgdct_batch_forward(sgdct_batch* b, const float* img, float* sino, void* stream) {
+  if (!b || !img || !sino) return fail(SGDCT_EINVAL, "null argument");
+  return batch_run(b, 0, img, sino, (hipStream_t)stream);
+}
+
+int sgdct_batch_backward(sgdct_batch* b, const float* sino, float* img, void* stream) {
+  if (!b || !img || !sino) return fail(SGDCT_EINVAL, "null argument");
+  return batch_run(b, 1, sino, img, (hipStream_t)stream);
+}
+
+int sgdct_batch_graph_launch(sgdct_batch* b, int direction, const float* in, float* out, void* stream) {
+  if (!b || !in || !out || (direction != 0 && direction != 1)) return fail(SGDCT_EINVAL, "bad graph arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) return fail(SGDCT_EINVAL, "graph capture needs a non-default stream");
+  sgdct_batch::Graph& g = b->graph[direction];
+  if (!g.exec || g.in != in || g.out != out) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = sgdct_batch::Graph{};
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "begin capture");
+    int rc = batch_run(b, direction, in, out, s);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s, &graph);
+    if (rc != SGDCT_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "end capture");
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g = sgdct_batch::Graph{};
+      return hip_fail(e, "graph instantiate");
+    }
+    g.in = in;
+    g.out = out;
+  }
+  hipError_t e = hipGraphLaunch(g.exec, s);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, "graph launch");
+}
 
 int sgdct_forward_dev(sgdct_projector* h, const float* img, float* sino, int batch, void* stream) {
   if (!h || !img || !sino || batch <= 0) return fail(SGDCT_EINVAL, "bad forward_dev arguments");

@@ -17,170 +17,17 @@
 #include <cmath>
 #include <cstdint>
 
-#include "footprint.hpp"
-#include "projector.hpp"
+#include "proj2d_dev.hpp"
 
 namespace sgdct {
 
-// ---------------------------------------------------------------------------
-// Basis traits: per-bin record, footprint of one (pixel, bin) pair.
-// ---------------------------------------------------------------------------
-template <int BASIS> struct Traits;
-
-template <> struct Traits<BASIS_B1> {
-  using Bin = B1Bin;
-  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
-    float sa = s_b1(ta, r.apb, r.b, r.a, r.k2, r.k4, r.cq, r.c2);
-    float sb = s_b1(tb, r.apb, r.b, r.a, r.k2, r.k4, r.cq, r.c2);
-    return mean_over(sa, sb, ta, tb);
-  }
-};
-
-template <> struct Traits<BASIS_B0> {
-  using Bin = B0Bin;
-  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
-    float sa = s_b0(ta, r.hpq, r.hmq, r.inv_p, r.inv_2pq);
-    float sb = s_b0(tb, r.hpq, r.hmq, r.inv_p, r.inv_2pq);
-    return mean_over(sa, sb, ta, tb);
-  }
-};
-
-// ---------------------------------------------------------------------------
-// A: bin-driven gather.
-// ---------------------------------------------------------------------------
 template <int BASIS>
 __global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float* __restrict__ img,
                                                       float* __restrict__ sino) {
-  using T = Traits<BASIS>;
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int v = wave / g.nchunk;
-  if (v >= g.np) return;
-  const int chunk = wave - v * g.nchunk;
-  const int u = chunk * 64 + lane;
-  img += (size_t)blockIdx.y * g.nx * g.ny;
-  sino += (size_t)blockIdx.y * g.np * g.nu;
-
-  const ViewGeo vg = g.views[v];
-  // Walk rows when the wave's middle ray is closer to the y axis, columns
-  // otherwise (wave-uniform choice).
-  const int umid = min(chunk * 64 + 32, g.nu - 1);
-  const double mx = vg.e0x + (umid + 0.5) * vg.pux, my = vg.e0y + (umid + 0.5) * vg.puy;
-  const bool rows = fabs(my) >= fabs(mx);
-  if (u >= g.nu) return;
-
-  const typename T::Bin rec = reinterpret_cast<const typename T::Bin*>(g.bins)[(size_t)v * g.nu + u];
-  // N_u and N_{u+1} as affine functions of the pixel indices.
-  const double eax = vg.e0x + u * vg.pux, eay = vg.e0y + u * vg.puy;
-  const double ebx = eax + vg.pux, eby = eay + vg.puy;
-  const double acx = -eay * g.dx, acy = eax * g.dy, ac0 = eax * vg.w0y - eay * vg.w0x;
-  const double bcx = -eby * g.dx, bcy = ebx * g.dy, bc0 = ebx * vg.w0y - eby * vg.w0x;
-  // Corner rule: bin u is in a pixel's range iff N_u + R_u >= 0 and
-  // N_{u+1} - R_{u+1} < 0 (max / min of N over the pixel's corners).  The
-  // reference clamps ranges into [0, nu-1] (layers/A_B1.cu:150-151), so bin 0
-  // also serves pixels projecting before the detector and bin nu-1 pixels
-  // projecting past it; one extra bin of margin covers every pixel whose
-  // footprint can still reach the end bins.
-  const double lax = (u == 0) ? eax - vg.pux : eax, lay = (u == 0) ? eay - vg.puy : eay;
-  const double lbx = (u == g.nu - 1) ? ebx + vg.pux : ebx, lby = (u == g.nu - 1) ? eby + vg.puy : eby;
-  const double qcx = -lay * g.dx, qcy = lax * g.dy, qc0 = lax * vg.w0y - lay * vg.w0x;
-  const double rcx = -lby * g.dx, rcy = lbx * g.dy, rc0 = lbx * vg.w0y - lby * vg.w0x;
-  const double ra = g.h * (fabs(qcx) + fabs(qcy)), rb = g.h * (fabs(rcx) + fabs(rcy));
-  // ta / tb step along the walk (edges u, u+1); pixel-set tests (edges q, r)
-  const double sa = rows ? acx : acy, ma = rows ? acy : acx;
-  const double sb = rows ? bcx : bcy, mb = rows ? bcy : bcx;
-  const double sq = rows ? qcx : qcy, mq = rows ? qcy : qcx;
-  const double sr = rows ? rcx : rcy, mr = rows ? rcy : rcx;
-  const int nmin = rows ? g.nx : g.ny, nmaj = rows ? g.ny : g.nx;
-  const int smin = rows ? 1 : g.nx, smaj = rows ? g.nx : 1;
-  const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
-  const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
-
-  // Lines crossed by the strip: between the extreme crossings of its two
-  // boundary lines with the first and last pixel of a line.
-  int m_lo = 0, m_hi = nmaj - 1;
-  if (mq != 0.0 && mr != 0.0) {
-    const double last = nmin - 1;
-    double c1 = (-ra - qc0) / mq, c2 = (-ra - qc0 - sq * last) / mq;
-    double c3 = (rb - rc0) / mr, c4 = (rb - rc0 - sr * last) / mr;
-    double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
-    lo = fmax(floor(lo) - 1.0, 0.0);
-    hi = fmin(ceil(hi) + 1.0, (double)(nmaj - 1));
-    m_lo = (int)lo;
-    m_hi = (int)hi;
-  }
-
-  // Pixel interval of line m: [ceil(lo(m)), floor(hi(m))], lo and hi affine
-  // in m (FP32).  sq and sr share a sign whenever the strip is within ~45
-  // degrees of the walk direction, which the per-wave orientation guarantees;
-  // otherwise the lane walks whole lines (outside the corner range B1's
-  // footprint is exactly zero).  Exact integer ties are not special-cased:
-  // the corner rule itself is discontinuous there.
-  float lo0 = 0.0f, lo1 = 0.0f, hi0 = (float)(nmin - 1), hi1 = 0.0f;
-  if (sq > 0.0 && sr > 0.0) {
-    lo0 = (float)((-ra - qc0) * isq);
-    lo1 = (float)(-mq * isq);
-    hi0 = (float)((rb - rc0) * isr);
-    hi1 = (float)(-mr * isr);
-  } else if (sq < 0.0 && sr < 0.0) {
-    hi0 = (float)((-ra - qc0) * isq);
-    hi1 = (float)(-mq * isq);
-    lo0 = (float)((rb - rc0) * isr);
-    lo1 = (float)(-mr * isr);
-  }
-  // ta, tb at a line's first pixel in FP64: ta = ka N_u = an * n + am * m + a0
-  const double an = (double)rec.ka * sa, am = (double)rec.ka * ma, a0 = (double)rec.ka * ac0;
-  const double bn = (double)rec.kb * sb, bm = (double)rec.kb * mb, b0 = (double)rec.kb * bc0;
-  const float lastn = (float)(nmin - 1);
-
-  double acc = 0.0;
-  for (int m = m_lo; m <= m_hi; ++m) {
-    const float fm = (float)m;
-    const float flo = fmaxf(ceilf(__builtin_fmaf(fm, lo1, lo0)), 0.0f);
-    const float fhi = fminf(floorf(__builtin_fmaf(fm, hi1, hi0)), lastn);
-    if (flo > fhi) continue;
-    const int n0 = (int)flo;
-    const double dm = (double)m, dn = (double)n0;
-    float ta = (float)__builtin_fma(dn, an, __builtin_fma(dm, am, a0));
-    float tb = (float)__builtin_fma(dn, bn, __builtin_fma(dm, bm, b0));
-    int idx = m * smaj + n0 * smin;
-    const int end = idx + ((int)fhi - n0 + 1) * smin;
-    float accl = 0.0f;
-    do {
-      accl = __builtin_fmaf(T::conv(rec, ta, tb), img[idx], accl);
-      idx += smin;
-      ta += dta;
-      tb += dtb;
-    } while (idx != end);
-    acc += (double)accl;
-  }
-  sino[(size_t)v * g.nu + u] = (float)(acc * g.dx * g.dy);
+  fwd_bin_wave<BASIS>(g, img + (size_t)blockIdx.y * g.nx * g.ny, sino + (size_t)blockIdx.y * g.np * g.nu,
+                      blockIdx.x * 4 + (threadIdx.x >> 6));
 }
 
-// ---------------------------------------------------------------------------
-// A^T: pixel-driven gather.
-// ---------------------------------------------------------------------------
-
-// Bin window of one view for a pixel tile: bins touched by the tile's box
-// widened by h, from the exact projection of its four corners, +-1 margin.
-__host__ __device__ inline void tile_window(const Proj2DArgs& g, const ViewGeo& vg, int ix0, int iy0, int* lo,
-                                            int* n, int cap = kMaxWindow) {
-  double umin = 1e300, umax = -1e300;
-  for (int c = 0; c < 4; ++c) {
-    double fx = (c & 1) ? (ix0 + kTile - 1 + g.h) : (ix0 - g.h);
-    double fy = (c & 2) ? (iy0 + kTile - 1 + g.h) : (iy0 - g.h);
-    double wx = fx * g.dx + vg.w0x, wy = fy * g.dy + vg.w0y;
-    double n0 = vg.e0x * wy - vg.e0y * wx;
-    double be = vg.pux * wy - vg.puy * wx;
-    double uu = -n0 / be;
-    umin = fmin(umin, uu);
-    umax = fmax(umax, uu);
-  }
-  double l = fmax(floor(umin) - 1.0, 0.0), hgh = fmin(floor(umax) + 1.0, (double)(g.nu - 1));
-  int li = (int)l, hi = (int)hgh;
-  *lo = li;
-  *n = hi >= li ? min(hi - li + 1, cap) : 0;
-}
 
 // Widest bin window over every tile and view (host-side, on the host copy of
 // the view table): the LDS-staged A^T kernel is used when it is <= kMaxWindow.
@@ -197,105 +44,15 @@ int max_tile_window(const Proj2DArgs& g, const ViewGeo* views_host) {
 }
 
 // Bin records are staged in LDS with the sinogram value in their first
-// padding word (B1Bin::pad0 / B0Bin::pad0).
-__device__ __forceinline__ void set_y(B1Bin& r, float y) { r.pad0 = y; }
-__device__ __forceinline__ void set_y(B0Bin& r, float y) { r.pad0 = y; }
-__device__ __forceinline__ float get_y(const B1Bin& r) { return r.pad0; }
-__device__ __forceinline__ float get_y(const B0Bin& r) { return r.pad0; }
 
-// USE_LDS: the host has checked that every tile's bin window fits in
-// kMaxWindow bins for every view (projector creation); otherwise records are
-// read from global memory (L2) directly.
 template <int BASIS, bool USE_LDS>
 __global__ __launch_bounds__(256) void bwd_pix_kernel(Proj2DArgs g, const float* __restrict__ sino,
                                                       float* __restrict__ img) {
-  using T = Traits<BASIS>;
-  using Bin = typename T::Bin;
-  static_assert(sizeof(Bin) % 16 == 0, "bin records are whole float4s");
-  __shared__ Bin lds[USE_LDS ? kViewChunk : 1][USE_LDS ? kMaxWindow : 1];
-  __shared__ int win_lo[kViewChunk], win_n[kViewChunk];
-
-  const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
-  const int ix0 = blockIdx.x * kTile, iy0 = blockIdx.y * kTile;
-  const int ix = ix0 + tx, iy = iy0 + ty;
-  sino += (size_t)blockIdx.z * g.np * g.nu;
-  img += (size_t)blockIdx.z * g.nx * g.ny;
-  const bool valid = ix < g.nx && iy < g.ny;
-  const Bin* bins = reinterpret_cast<const Bin*>(g.bins);
-  const double fx = ix * g.dx, fy = iy * g.dy;
-  const float last = (float)(g.nu - 1);
-
-  double acc = 0.0;
-  for (int v0 = 0; v0 < g.np; v0 += kViewChunk) {
-    const int nv = min(kViewChunk, g.np - v0);
-    if (USE_LDS) {
-      if ((int)threadIdx.x < nv) {
-        int lo, n;
-        tile_window(g, g.views[v0 + threadIdx.x], ix0, iy0, &lo, &n);
-        win_lo[threadIdx.x] = lo;
-        win_n[threadIdx.x] = n;
-      }
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < nv * kMaxWindow; idx += blockDim.x) {
-        const int j = idx / kMaxWindow, i = idx - j * kMaxWindow;
-        if (i < win_n[j]) {
-          const size_t gi = (size_t)(v0 + j) * g.nu + win_lo[j] + i;
-          Bin r = bins[gi];
-          set_y(r, sino[gi]);
-          lds[j][i] = r;
-        }
-      }
-      __syncthreads();
-    }
-    if (valid) {
-      for (int j = 0; j < nv; ++j) {
-        const int v = v0 + j;
-        const ViewGeo& vg = g.views[v];
-        const double wx = fx + vg.w0x, wy = fy + vg.w0y;
-        const double n0 = vg.e0x * wy - vg.e0y * wx;
-        const double be = vg.pux * wy - vg.puy * wx;
-        const float bef = (float)be;
-        const float kc = -(float)n0 * __builtin_amdgcn_rcpf(bef);
-        const float gx = __builtin_fmaf(kc, vg.gx1, vg.gx0), gy = __builtin_fmaf(kc, vg.gy1, vg.gy0);
-        const float hw = g.h * (__builtin_fabsf(gx) + __builtin_fabsf(gy)) *
-                         __builtin_amdgcn_rcpf(__builtin_fabsf(bef) - vg.db);
-        // clamped corner range (layers/A_B1.cu:150-151): pixels projecting
-        // wholly before / past the detector still visit bin 0 / nu-1
-        float flo = fminf(fmaxf(floorf(kc - hw), 0.0f), last);
-        float fhi = fminf(fmaxf(floorf(kc + hw), 0.0f), last);
-        if (USE_LDS) {
-          // The window holds every bin of the exact corner range of the
-          // tile (+-1); bins of the rounded range outside it carry a zero
-          // footprint, so clamping keeps the LDS reads inside the window.
-          flo = fmaxf(flo, (float)win_lo[j]);
-          fhi = fminf(fhi, (float)(win_lo[j] + win_n[j] - 1));
-        }
-        const int ulo = (int)flo, cnt = (int)fhi - ulo + 1;
-        float na = (float)__builtin_fma((double)ulo, be, n0);
-        float accv = 0.0f;
-        if (USE_LDS) {
-          const Bin* rp = &lds[j][ulo - win_lo[j]];
-          for (int k = 0; k < cnt; ++k) {
-            const Bin rec = rp[k];
-            const float nb = na + bef;
-            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), get_y(rec), accv);
-            na = nb;
-          }
-        } else {
-          const size_t base = (size_t)v * g.nu + ulo;
-          for (int k = 0; k < cnt; ++k) {
-            const Bin rec = bins[base + k];
-            const float nb = na + bef;
-            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), sino[base + k], accv);
-            na = nb;
-          }
-        }
-        acc += (double)accv;
-      }
-    }
-    if (USE_LDS) __syncthreads();
-  }
-  if (valid) img[(size_t)iy * g.nx + ix] = (float)(acc * g.dx * g.dy);
+  using Bin = typename Traits<BASIS>::Bin;
+  __shared__ Bin lds[USE_LDS ? kViewChunk : 1][kMaxWindow];
+  __shared__ int win[2 * kViewChunk];
+  bwd_pix_tile<BASIS, USE_LDS>(g, sino + (size_t)blockIdx.z * g.np * g.nu, img + (size_t)blockIdx.z * g.nx * g.ny,
+                               blockIdx.x, blockIdx.y, lds, win);
 }
 
 // ---------------------------------------------------------------------------
@@ -365,3 +122,4 @@ hipError_t launch_count(const Proj2DArgs& g, unsigned long long* out, hipStream_
 }
 
 }  // namespace sgdct
+

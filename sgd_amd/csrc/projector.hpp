@@ -55,6 +55,28 @@ struct Cone3DArgs {
   double du;        // |pu|
 };
 
+// One slice of a batched persistent launch (batch.hip): a 2-D projector's
+// arguments and its images' offsets in the packed batch buffers.
+struct SliceDesc {
+  Proj2DArgs g;
+  int basis;               // BASIS_B0, BASIS_B1 or BASIS_SF
+  int lds;                 // A^T may stage bin windows in LDS (B0 / B1)
+  long long img_off, sino_off;
+};
+
+// Device-side work queue of a slice batch: slices in processing order and the
+// cumulative item counts (A: 64-bin waves, A^T: 16x16 pixel tiles).
+struct BatchPlan {
+  const SliceDesc* slices;
+  const int* fwd_order;
+  const long long* fwd_prefix;  // [nslice + 1]
+  const int* bwd_order;
+  const long long* bwd_prefix;  // [nslice + 1]
+  unsigned int* counters;       // [2] queue heads (A, A^T)
+  int nslice;
+  long long nfwd, nbwd;
+};
+
 constexpr int kMaxWindow = 128;   // bins of one view cached in LDS per pixel tile
 constexpr int kViewChunk = 4;     // views per LDS refill in the gather kernels
 constexpr int kTile = 16;         // pixel tile edge of the gather kernels

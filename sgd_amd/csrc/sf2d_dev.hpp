@@ -1,0 +1,181 @@
+// sf2d_dev.hpp -- device code of the 2-D separable-footprint projector,
+// shared by sf2d.hip and the batched persistent kernels (batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "projector.hpp"
+
+namespace sgdct {
+
+struct Trap {
+  float q0, q1, q2, q3, ir, iff;  // sorted corners, 1/(2(q1-q0)), 1/(2(q3-q2))
+};
+
+__device__ __forceinline__ void sort4(float& a, float& b, float& c, float& d) {
+  float t;
+  t = fminf(a, b); b = fmaxf(a, b); a = t;
+  t = fminf(c, d); d = fmaxf(c, d); c = t;
+  t = fminf(a, c); c = fmaxf(a, c); a = t;
+  t = fminf(b, d); d = fmaxf(b, d); b = t;
+  t = fminf(b, c); c = fmaxf(b, c); b = t;
+}
+
+__device__ __forceinline__ Trap make_trap(float q0, float q1, float q2, float q3) {
+  sort4(q0, q1, q2, q3);
+  Trap t;
+  t.q0 = q0; t.q1 = q1; t.q2 = q2; t.q3 = q3;
+  t.ir = __builtin_amdgcn_rcpf(2.0f * (q1 - q0));
+  t.iff = __builtin_amdgcn_rcpf(2.0f * (q3 - q2));
+  return t;
+}
+
+// Area of the unit-height trapezoid over [s1, s2] (layers/A_SF.cu:48-84).
+__device__ __forceinline__ float trap_gamma(const Trap& t, float s1, float s2) {
+  float b1 = fmaxf(s1, t.q0), b2 = fminf(s2, t.q1);
+  float d0 = b2 - t.q0, d1 = b1 - t.q0;
+  float g = (b2 > b1) ? (d0 * d0 - d1 * d1) * t.ir : 0.0f;
+  b1 = fmaxf(s1, t.q1);
+  b2 = fminf(s2, t.q2);
+  g += fmaxf(b2 - b1, 0.0f);
+  b1 = fmaxf(s1, t.q2);
+  b2 = fminf(s2, t.q3);
+  float e1 = b1 - t.q3, e2 = b2 - t.q3;
+  g += (b2 > b1) ? (e1 * e1 - e2 * e2) * t.iff : 0.0f;
+  return g;
+}
+
+__device__ __forceinline__ float sf_weight(float dx, float wx, float wy) {
+  float r2 = wx * wx + wy * wy;
+  return dx * __builtin_sqrtf(r2) * __builtin_amdgcn_rcpf(fmaxf(fabsf(wx), fabsf(wy)));
+}
+
+// A: bin-driven gather (same strip walk as fwd_bin_kernel, corner rule h = 1/2).
+// One wave: bins [64 chunk, 64 chunk + 64) of view v, wave = v * nchunk + chunk.
+__device__ __forceinline__ void sf_fwd_wave(const Proj2DArgs& g, const float* __restrict__ img,
+                                            float* __restrict__ sino, int wave) {
+  const int lane = threadIdx.x & 63;
+  const int v = wave / g.nchunk;
+  if (v >= g.np) return;
+  const int chunk = wave - v * g.nchunk;
+  const int u = chunk * 64 + lane;
+  const ViewGeo vg = g.views[v];
+  const int umid = min(chunk * 64 + 32, g.nu - 1);
+  const double mx = vg.e0x + (umid + 0.5) * vg.pux, my = vg.e0y + (umid + 0.5) * vg.puy;
+  const bool rows = fabs(my) >= fabs(mx);
+  if (u >= g.nu) return;
+
+  const double eax = vg.e0x + u * vg.pux, eay = vg.e0y + u * vg.puy;
+  const double ebx = eax + vg.pux, eby = eay + vg.puy;
+  const double acx = -eay * g.dx, acy = eax * g.dy, ac0 = eax * vg.w0y - eay * vg.w0x;
+  const double bcx = -eby * g.dx, bcy = ebx * g.dy, bc0 = ebx * vg.w0y - eby * vg.w0x;
+  const double ra = 0.5 * (fabs(acx) + fabs(acy)), rb = 0.5 * (fabs(bcx) + fabs(bcy));
+  const double sa = rows ? acx : acy, ma = rows ? acy : acx;
+  const double sb = rows ? bcx : bcy, mb = rows ? bcy : bcx;
+  const int nmin = rows ? g.nx : g.ny, nmaj = rows ? g.ny : g.nx;
+  const size_t smin = rows ? 1 : g.nx, smaj = rows ? g.nx : 1;
+  const double isa = sa != 0.0 ? 1.0 / sa : 0.0, isb = sb != 0.0 ? 1.0 / sb : 0.0;
+  // beta(ix, iy) = cross(pu, w): affine, gradient (gx1, gy1) per index step
+  const double bcxx = -vg.puy * g.dx, bcyy = vg.pux * g.dy, bc00 = vg.pux * vg.w0y - vg.puy * vg.w0x;
+  const float dN = (float)sa, dB = (float)(rows ? bcxx : bcyy);
+  const float dwx = rows ? (float)g.dx : 0.0f, dwy = rows ? 0.0f : (float)g.dy;
+  // corner offsets (+-1/2, +-1/2) of N_u and beta
+  const float nx_ = 0.5f * (float)acx, ny_ = 0.5f * (float)acy;
+  const float bx_ = 0.5f * (float)bcxx, by_ = 0.5f * (float)bcyy;
+  const float fdx = (float)g.dx;
+
+  int m_lo = 0, m_hi = nmaj - 1;
+  if (ma != 0.0 && mb != 0.0) {
+    const double last = nmin - 1;
+    double c1 = (-ra - ac0) / ma, c2 = (-ra - ac0 - sa * last) / ma;
+    double c3 = (rb - bc0) / mb, c4 = (rb - bc0 - sb * last) / mb;
+    double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
+    m_lo = (int)fmax(floor(lo) - 1.0, 0.0);
+    m_hi = (int)fmin(ceil(hi) + 1.0, (double)(nmaj - 1));
+  }
+  double acc = 0.0;
+  for (int m = m_lo; m <= m_hi; ++m) {
+    const double r1 = -ra - ma * m - ac0;
+    const double r2 = rb - mb * m - bc0;
+    double lo = 0.0, hi = (double)(nmin - 1);
+    if (sa > 0.0) lo = fmax(lo, ceil(r1 * isa));
+    else if (sa < 0.0) hi = fmin(hi, floor(r1 * isa));
+    else if (r1 > 0.0) continue;
+    if (sb > 0.0) hi = fmin(hi, ceil(r2 * isb) - 1.0);
+    else if (sb < 0.0) lo = fmax(lo, floor(r2 * isb) + 1.0);
+    else if (r2 <= 0.0) continue;
+    if (lo > hi) continue;
+    const int n0 = (int)lo, n1 = (int)hi;
+    const int ixs = rows ? n0 : m, iys = rows ? m : n0;
+    float nu_ = (float)(acx * ixs + acy * iys + ac0);
+    float be = (float)(bcxx * ixs + bcyy * iys + bc00);
+    float wx = (float)(ixs * g.dx + vg.w0x), wy = (float)(iys * g.dy + vg.w0y);
+    const float* p = img + (size_t)m * smaj + (size_t)n0 * smin;
+    float accl = 0.0f;
+    for (int n = n0; n <= n1; ++n) {
+      // corner projections relative to edge u: -N_u(c) / beta(c)
+      const float q0 = -(nu_ - nx_ - ny_) * __builtin_amdgcn_rcpf(be - bx_ - by_);
+      const float q1 = -(nu_ - nx_ + ny_) * __builtin_amdgcn_rcpf(be - bx_ + by_);
+      const float q2 = -(nu_ + nx_ - ny_) * __builtin_amdgcn_rcpf(be + bx_ - by_);
+      const float q3 = -(nu_ + nx_ + ny_) * __builtin_amdgcn_rcpf(be + bx_ + by_);
+      const Trap t = make_trap(q0, q1, q2, q3);
+      const float gm = trap_gamma(t, 0.0f, 1.0f);
+      const float val = (gm > 1e-7f) ? gm * sf_weight(fdx, wx, wy) : 0.0f;
+      accl = __builtin_fmaf(val, *p, accl);
+      p += smin;
+      nu_ += dN;
+      be += dB;
+      wx += dwx;
+      wy += dwy;
+    }
+    acc += (double)accl;
+  }
+  sino[(size_t)v * g.nu + u] = (float)acc;
+}
+
+
+// A^T: pixel-driven gather, one 16x16 tile per 256-thread block.
+__device__ __forceinline__ void sf_bwd_tile(const Proj2DArgs& g, const float* __restrict__ sino,
+                                            float* __restrict__ img, int tile_x, int tile_y) {
+  const int ix = tile_x * kTile + (threadIdx.x & (kTile - 1));
+  const int iy = tile_y * kTile + threadIdx.x / kTile;
+  if (ix >= g.nx || iy >= g.ny) return;
+  const float fdx = (float)g.dx;
+  double acc = 0.0;
+  for (int v = 0; v < g.np; ++v) {
+    const ViewGeo& vg = g.views[v];
+    const double wx = ix * g.dx + vg.w0x, wy = iy * g.dy + vg.w0y;
+    const double n0 = vg.e0x * wy - vg.e0y * wx;
+    const double be = vg.pux * wy - vg.puy * wx;
+    const double kc = -n0 / be;
+    const double kfl = floor(kc);
+    const float kf = (float)(kc - kfl);
+    const float kcf = (float)kc, bef = (float)be;
+    const float gx = 0.5f * __builtin_fmaf(kcf, vg.gx1, vg.gx0), gy = 0.5f * __builtin_fmaf(kcf, vg.gy1, vg.gy0);
+    const float bx = 0.5f * vg.gx1, by = 0.5f * vg.gy1;
+    // corner c: k*(c) - kc = -(d.G) / (beta + d.grad beta)
+    const float q0 = kf + (gx + gy) * __builtin_amdgcn_rcpf(bef - bx - by);
+    const float q1 = kf + (gx - gy) * __builtin_amdgcn_rcpf(bef - bx + by);
+    const float q2 = kf - (gx - gy) * __builtin_amdgcn_rcpf(bef + bx - by);
+    const float q3 = kf - (gx + gy) * __builtin_amdgcn_rcpf(bef + bx + by);
+    const Trap t = make_trap(q0, q1, q2, q3);
+    const int ki = (int)kfl;
+    const int jlo = (int)floorf(t.q0), jhi = (int)floorf(t.q3);
+    if (ki + jhi < 0 || ki + jlo > g.nu - 1) continue;
+    const int ulo = max(ki + jlo, 0), uhi = min(ki + jhi, g.nu - 1);
+    const float w = sf_weight(fdx, (float)wx, (float)wy);
+    const float* row = sino + (size_t)v * g.nu;
+    float accv = 0.0f;
+    for (int u = ulo; u <= uhi; ++u) {
+      const float s1 = (float)(u - ki);
+      const float gm = trap_gamma(t, s1, s1 + 1.0f);
+      accv = __builtin_fmaf((gm > 1e-7f) ? gm : 0.0f, row[u], accv);
+    }
+    acc += (double)(accv * w);
+  }
+  img[(size_t)iy * g.nx + ix] = (float)acc;
+}
+
+
+}  // namespace sgdct

@@ -1,0 +1,83 @@
+"""Explicit system matrices (pixel_matrix_generation / bilinear_matrix_generation,
+layers/A_B0.cu:449-492, A_B1.cu:545-593) from the GPU against the oracle's
+restatement of the reference's matrix kernels (or_matrix_generation):
+  * same COO triplets in the reference's order (view, bin, i, j), values to
+    <= 1e-5 relative Frobenius norm, counts within 0.5 %;
+  * B1 rows are the footprint of the B1 projector: matrix @ x * dx * dy equals
+    b1_forward_projection on the same views (the reference's own matrix kernel
+    is one bin off, see the oracle);
+  * a buffer smaller than the triplet count is an error, not a crash.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = {
+    "small": (8, 60, 24, 20, 1.0, 1.0, 1.2, 200.0, 100.0),
+    "medium": (32, 171, 50, 50, 1.0, 1.0, 0.78125, 156.25, 78.125),
+}
+
+
+def _gpu_matrix(basis, geom, cap=None):
+    from sgd_amd import _lib
+    np_, nu, nx, ny, dx, dy, du, lsd, lso = geom
+    L = _lib.lib()
+    f = L.bilinear_matrix_generation if basis == "B1" else L.pixel_matrix_generation
+    cap = cap if cap is not None else np_ * nu * nx * ny // 4
+    buf = np.zeros((max(cap, 1), 3))
+    n = f(np_, nu, nx, ny, dx, dy, du, lsd, lso, _lib.dptr(buf), cap)
+    return n, buf[:max(n, 0)]
+
+
+def _dense(rows, cols, vals, geom):
+    np_, nu, nx, ny = geom[:4]
+    M = np.zeros((np_ * nu, nx * ny))
+    M[rows, cols] = vals
+    return M
+
+
+@pytest.mark.parametrize("geom", sorted(GEOMS))
+@pytest.mark.parametrize("basis", ["B0", "B1"])
+def test_matrix_matches_oracle(cuda, oracle_exact, basis, geom):
+    from pyoracle import matrix_generation
+    print("matrix", basis, geom, flush=True)
+    g = GEOMS[geom]
+    n, trip = _gpu_matrix(basis, g)
+    assert n > 0
+    r, c, v = trip[:, 0].astype(np.int64), trip[:, 1].astype(np.int64), trip[:, 2]
+    key = r * (g[2] * g[3]) + c
+    assert np.all(np.diff(key) > 0), "triplets out of the reference's order"
+    assert np.all(v > 0)
+    ro, co, vo = matrix_generation(oracle_exact, basis, *g, fixed=True)
+    M, Mo = _dense(r, c, v, g), _dense(ro, co, vo, g)
+    err = np.linalg.norm(M - Mo) / np.linalg.norm(Mo)
+    print("%s %s: nnz gpu %d oracle %d, rel %.2e" % (basis, geom, n, len(vo), err))
+    assert err <= 1e-5
+    assert abs(n - len(vo)) <= 0.005 * len(vo)
+
+
+def test_b1_matrix_is_the_projector(cuda):
+    import torch
+    from sgd_amd.torch_ops import Projector
+    print("matrix vs projector", flush=True)
+    g = GEOMS["medium"]
+    np_, nu, nx, ny, dx, dy, du, lsd, lso = g
+    n, trip = _gpu_matrix("B1", g)
+    M = _dense(trip[:, 0].astype(np.int64), trip[:, 1].astype(np.int64), trip[:, 2], g)
+    ang = np.arange(np_) * ((2 * np.pi * (np_ - 1) / np_) / (np_ - 1))
+    P = Projector("B1", nx, ny, ang, nu, dx, dy, du, lsd, lso)
+    x = np.random.default_rng(0).random((ny, nx))
+    y = P.forward(torch.tensor(x, dtype=torch.float32, device=cuda)).double().cpu().numpy().reshape(-1)
+    ym = M @ x.T.reshape(-1) * dx * dy  # columns are ix*ny + iy
+    err = np.linalg.norm(ym - y) / np.linalg.norm(y)
+    print("B1 matrix @ x vs projector rel %.2e" % err)
+    assert err <= 1e-5
+
+
+def test_buffer_too_small_is_an_error(cuda):
+    from sgd_amd import _lib
+    g = GEOMS["small"]
+    n, _ = _gpu_matrix("B0", g)
+    n2, _ = _gpu_matrix("B0", g, cap=n - 1)
+    assert n2 < 0 and "needed" in _lib.last_error()

@@ -6,8 +6,8 @@ is empty).  Pins:
   * config 1 (B0, 256^2, 360 views, 512 bins, noise-free): 20 SIRT
     iterations decrease the data residual monotonically and move the image
     towards the phantom;
-  * ASD-POCS lowers TV relative to SIRT at equal iterations and stops at the
-    requested relative residual.
+  * ASD-POCS lowers TV relative to SIRT at equal iterations on noisy data,
+    and stops at the requested relative residual on consistent data.
 """
 import numpy as np
 import pytest
@@ -74,6 +74,9 @@ def test_asd_pocs_reduces_tv_and_stops_at_tolerance(cuda):
     r = asd_pocs(P, b, niter=30, weights=w)
     print("TV sirt %.3e asd %.3e" % (float(tv(xs)), float(tv(r.x))))
     assert float(tv(r.x)) < float(tv(xs))
-    r2 = asd_pocs(P, b, niter=500, tol=2e-2, weights=w)
+    # the tolerance stop on consistent data (the noisy data's residual floor is
+    # its own noise level, ~9e-2 here)
+    b0 = torch.tensor(y, dtype=torch.float32, device=cuda)
+    r2 = asd_pocs(P, b0, niter=500, tol=2e-2, weights=w)
     print("asd to tol: iters %d last res %.3e" % (r2.iters, r2.residuals[-1]))
     assert r2.residuals[-1] <= 2e-2 and r2.iters < 500
