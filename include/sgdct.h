@@ -141,6 +141,8 @@ int sgdct_sirt_residual(const float *b, const float *ax, const float *w, float *
 int sgdct_sirt_update(float *x, const float *c, const float *v, float lam, int positivity, long long n,
                       long long per, void *stream);
 int sgdct_axpy(float *x, const float *g, float a, long long n, void *stream);
+/* x[k] += a[k / per] g[k], a a device array of n / per coefficients */
+int sgdct_axpy_batch(float *x, const float *g, const float *a, long long n, long long per, void *stream);
 
 /* ---- batched slices (no reference counterpart: the reference projects one
  * slice per call and launches one kernel per view, layers/A_B1.cu:512-541) --
@@ -165,6 +167,48 @@ int sgdct_batch_backward(sgdct_batch *b, const float *sino, float *img, void *st
  * `stream` (non-default) at the first call for these buffers and replayed
  * afterwards; the graph clears its queue counters with a kernel, not a memset. */
 int sgdct_batch_graph_launch(sgdct_batch *b, int direction, const float *in, float *out, void *stream);
+
+/* ---- warm-start network kernels (no reference counterpart; the reference's
+ * network code is a cuBLAS MLP, network.cu and layers/linear.cu).  NHWC
+ * bfloat16 activations (passed as void*), FP32 gradients of activations,
+ * weights bfloat16 W[cout][9 cin] (k = (3 ky + kx) cin + c), FP32 master
+ * weights.  All asynchronous on `stream`; workspaces are caller-owned device
+ * FP32 buffers of at least the size the matching *_workspace call returns. */
+/* out = relu?(conv3x3(in, w) + bias), zero padding; bf16 out, or FP32 when
+ * out_f32 (the input-gradient pass: in = bf16 output gradient, w = flipped). */
+int sgdct_nn_conv3x3(const void *in, const void *w, const float *bias, void *out, int B, int H, int W, int cin,
+                     int cout, int relu, int out_f32, void *stream);
+long long sgdct_nn_wgrad_workspace(int B, int H, int W, int cin, int cout);
+/* dw[co][k] = sum over pixels of dy[m][co] x[m + tap(k)][c(k)] */
+int sgdct_nn_wgrad(const void *x, const void *dy, float *dw, float *workspace, long long workspace_floats, int B,
+                   int H, int W, int cin, int cout, void *stream);
+long long sgdct_nn_colsum_workspace(long long M, int C);
+int sgdct_nn_colsum(const void *a, float *out, float *workspace, long long workspace_floats, long long M, int C,
+                    void *stream);
+int sgdct_nn_pad_input(const float *x, void *out8, long long npix, void *stream);
+/* dst (B, H, W, C) = 2x2 average of src (B, 2H, 2W, C); back: gsrc += dv / 4 */
+int sgdct_nn_pool2(const void *src, void *dst, int B, int H, int W, int C, void *stream);
+int sgdct_nn_pool2_back(const float *dv, float *gsrc, int B, int H, int W, int C, void *stream);
+/* dst (B, H, W, C0 + C1) = [up2(low (B, H/2, W/2, C0)), skip (B, H, W, C1)];
+ * back: glow = sum over each 2x2 block of dv[..., :C0], gskip += dv[..., C0:] */
+int sgdct_nn_upcat(const void *low, const void *skip, void *dst, int B, int H, int W, int C0, int C1, void *stream);
+int sgdct_nn_upcat_back(const float *dv, float *glow, float *gskip, int B, int H, int W, int C0, int C1,
+                        void *stream);
+/* y = x + d0 . w + b (1x1 output head); back: dy = 2 (y - target) / M,
+ * gd0 = dy w, gw[0:C] = sum dy d0, gw[C] = sum dy */
+int sgdct_nn_head(const void *d0, const float *w, const float *b, const float *x, float *y, long long M, int C,
+                  void *stream);
+long long sgdct_nn_head_workspace(long long M, int C);
+int sgdct_nn_head_back(const float *y, const float *target, const float *w, const void *d0, float *dy, float *gd0,
+                       float *gw, float *workspace, long long workspace_floats, long long M, int C, void *stream);
+/* out = bf16(g * (act > 0)) */
+int sgdct_nn_relu_mask(const float *g, const void *act, void *out, long long n, void *stream);
+/* wt[c][t][o] = w[o][8 - t][c] */
+int sgdct_nn_flip_weights(const void *w, void *wt, int cin, int cout, void *stream);
+int sgdct_nn_adam(float *w32, void *w16, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,
+                  float eps, int step, void *stream);
+int sgdct_nn_fill(float *p, long long n, float value, void *stream);
+int sgdct_nn_to_bf16(const float *src, void *dst, long long n, void *stream);
 
 /* Human-readable description of the calling thread's last error. */
 const char *sgdct_last_error(void);

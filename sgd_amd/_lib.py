@@ -71,6 +71,32 @@ def lib():
     L.b1_3d_forward_projection.restype = ctypes.c_ulonglong
     L.b1_3d_backward_projection.argtypes = [_dp, _dp, _h]
     L.b1_3d_backward_projection.restype = ctypes.c_int
+    _v, _i, _ll, _f = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
+    sigs = {
+        "sgdct_nn_conv3x3": [_v, _v, _v, _v] + [_i] * 7 + [_v],
+        "sgdct_nn_wgrad_workspace": [_i] * 5,
+        "sgdct_nn_wgrad": [_v, _v, _v, _v, _ll] + [_i] * 5 + [_v],
+        "sgdct_nn_colsum_workspace": [_ll, _i],
+        "sgdct_nn_colsum": [_v, _v, _v, _ll, _ll, _i, _v],
+        "sgdct_nn_pad_input": [_v, _v, _ll, _v],
+        "sgdct_nn_pool2": [_v, _v] + [_i] * 4 + [_v],
+        "sgdct_nn_pool2_back": [_v, _v] + [_i] * 4 + [_v],
+        "sgdct_nn_upcat": [_v, _v, _v] + [_i] * 5 + [_v],
+        "sgdct_nn_upcat_back": [_v, _v, _v] + [_i] * 5 + [_v],
+        "sgdct_nn_head": [_v] * 5 + [_ll, _i, _v],
+        "sgdct_nn_head_workspace": [_ll, _i],
+        "sgdct_nn_head_back": [_v] * 8 + [_ll, _ll, _i, _v],
+        "sgdct_nn_relu_mask": [_v, _v, _v, _ll, _v],
+        "sgdct_nn_flip_weights": [_v, _v, _i, _i, _v],
+        "sgdct_nn_adam": [_v] * 5 + [_ll, _f, _f, _f, _f, _i, _v],
+        "sgdct_nn_fill": [_v, _ll, _f, _v],
+        "sgdct_nn_to_bf16": [_v, _v, _ll, _v],
+        "sgdct_axpy_batch": [_v, _v, _v, _ll, _ll, _v],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _ll if name.endswith("_workspace") else _i
     for name in ("pixel_matrix_generation", "bilinear_matrix_generation"):
         f = getattr(L, name)
         f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [_dp, ctypes.c_int]

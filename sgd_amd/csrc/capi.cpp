@@ -45,6 +45,30 @@ constexpr int kMaxNz3D = 512, kMaxNv3D = 512;
 hipError_t launch_matgen(int basis, const Proj2DArgs& g, bool write, unsigned int* counts, const long long* offsets,
                          double* out, hipStream_t s);
 hipError_t launch_batch_forward(const BatchPlan& P, int blocks, const float* img, float* sino, hipStream_t s);
+hipError_t nn_conv3x3(const void* in, const void* w, const float* bias, void* out, int B, int H, int W, int cin,
+                      int cout, int relu, int out_f32, hipStream_t s);
+long long nn_wgrad_splits(long long M);
+hipError_t nn_wgrad(const void* x, const void* dy, float* dw, float* partial, int B, int H, int W, int cin, int cout,
+                    hipStream_t s);
+hipError_t nn_colsum_bf16(const void* a, float* out, float* partial, long long M, int C, hipStream_t s);
+hipError_t nn_pad_input(const float* x, void* out8, long long n, hipStream_t s);
+hipError_t nn_pool2(const void* src, void* dst, int B, int H, int W, int C, hipStream_t s);
+hipError_t nn_pool2_back(const float* dv, float* gsrc, int B, int H, int W, int C, hipStream_t s);
+hipError_t nn_upcat(const void* low, const void* skip, void* dst, int B, int H, int W, int C0, int C1,
+                    hipStream_t s);
+hipError_t nn_upcat_back(const float* dv, float* glow, float* gskip, int B, int H, int W, int C0, int C1,
+                         hipStream_t s);
+hipError_t nn_head(const void* d0, const float* w, const float* b, const float* x, float* y, long long M, int C,
+                   hipStream_t s);
+hipError_t nn_head_back(const float* y, const float* target, const float* w, const void* d0, float* dy, float* gd0,
+                        float* gw, float* partial, long long M, int C, hipStream_t s);
+hipError_t nn_relu_mask(const float* g, const void* act, void* out, long long n, hipStream_t s);
+hipError_t nn_flip_weights(const void* w, void* wt, int cin, int cout, hipStream_t s);
+hipError_t nn_adam(float* w32, void* w16, const float* g, float* m, float* v, long long n, float lr, float b1,
+                   float b2, float eps, int step, hipStream_t s);
+hipError_t nn_fill(float* p, long long n, float value, hipStream_t s);
+hipError_t nn_to_bf16(const float* src, void* dst, long long n, hipStream_t s);
+hipError_t launch_axpy_batch(float* x, const float* g, const float* a, size_t n, size_t per, hipStream_t s);
 hipError_t launch_batch_backward(const BatchPlan& P, int blocks, const float* sino, float* img, hipStream_t s);
 }  // namespace sgdct
 
@@ -884,6 +908,124 @@ int sgdct_shape(const sgdct_projector* h, int* out) {
 long long sgdct_eval_count(sgdct_projector* h) {
   if (!h) return fail(SGDCT_EINVAL, "null handle");
   return eval_count(h);
+}
+
+// ---- warm-start network kernels (device pointers, asynchronous) ----------
+namespace {
+int nn_ret(hipError_t e, const char* what) { return e == hipSuccess ? SGDCT_OK : hip_fail(e, what); }
+bool dims_ok(int B, int H, int W) { return B > 0 && H > 0 && W > 0; }
+}  // namespace
+
+int sgdct_nn_conv3x3(const void* in, const void* w, const float* bias, void* out, int B, int H, int W, int cin,
+                     int cout, int relu, int out_f32, void* stream) {
+  if (!in || !w || !out || !dims_ok(B, H, W) || cin <= 0 || cin % 8 || cout <= 0)
+    return fail(SGDCT_EINVAL, "bad conv3x3 arguments (cin must be a multiple of 8)");
+  return nn_ret(nn_conv3x3(in, w, bias, out, B, H, W, cin, cout, relu, out_f32, (hipStream_t)stream), "conv3x3");
+}
+
+long long sgdct_nn_wgrad_workspace(int B, int H, int W, int cin, int cout) {
+  if (!dims_ok(B, H, W) || cin <= 0 || cout <= 0) return fail(SGDCT_EINVAL, "bad wgrad shape");
+  return nn_wgrad_splits((long long)B * H * W) * cout * 9LL * cin;
+}
+
+int sgdct_nn_wgrad(const void* x, const void* dy, float* dw, float* workspace, long long workspace_floats, int B,
+                   int H, int W, int cin, int cout, void* stream) {
+  if (!x || !dy || !dw || !workspace || !dims_ok(B, H, W) || cin <= 0 || cin % 8 || cout <= 0 || cout % 8)
+    return fail(SGDCT_EINVAL, "bad wgrad arguments (cin and cout must be multiples of 8)");
+  if (workspace_floats < sgdct_nn_wgrad_workspace(B, H, W, cin, cout))
+    return fail(SGDCT_EINVAL, "wgrad workspace too small");
+  return nn_ret(nn_wgrad(x, dy, dw, workspace, B, H, W, cin, cout, (hipStream_t)stream), "wgrad");
+}
+
+long long sgdct_nn_colsum_workspace(long long M, int C) {
+  if (M <= 0 || C <= 0) return fail(SGDCT_EINVAL, "bad colsum shape");
+  return (M + 4095) / 4096 * (long long)C;
+}
+
+int sgdct_nn_colsum(const void* a, float* out, float* workspace, long long workspace_floats, long long M, int C,
+                    void* stream) {
+  if (!a || !out || !workspace || M <= 0 || C <= 0) return fail(SGDCT_EINVAL, "bad colsum arguments");
+  if (workspace_floats < sgdct_nn_colsum_workspace(M, C)) return fail(SGDCT_EINVAL, "colsum workspace too small");
+  return nn_ret(nn_colsum_bf16(a, out, workspace, M, C, (hipStream_t)stream), "colsum");
+}
+
+int sgdct_nn_pad_input(const float* x, void* out8, long long npix, void* stream) {
+  if (!x || !out8 || npix <= 0) return fail(SGDCT_EINVAL, "bad pad_input arguments");
+  return nn_ret(nn_pad_input(x, out8, npix, (hipStream_t)stream), "pad_input");
+}
+
+int sgdct_nn_pool2(const void* src, void* dst, int B, int H, int W, int C, void* stream) {
+  if (!src || !dst || !dims_ok(B, H, W) || C <= 0) return fail(SGDCT_EINVAL, "bad pool2 arguments");
+  return nn_ret(nn_pool2(src, dst, B, H, W, C, (hipStream_t)stream), "pool2");
+}
+
+int sgdct_nn_pool2_back(const float* dv, float* gsrc, int B, int H, int W, int C, void* stream) {
+  if (!dv || !gsrc || !dims_ok(B, H, W) || C <= 0) return fail(SGDCT_EINVAL, "bad pool2_back arguments");
+  return nn_ret(nn_pool2_back(dv, gsrc, B, H, W, C, (hipStream_t)stream), "pool2_back");
+}
+
+int sgdct_nn_upcat(const void* low, const void* skip, void* dst, int B, int H, int W, int C0, int C1, void* stream) {
+  if (!low || !skip || !dst || !dims_ok(B, H, W) || H % 2 || W % 2 || C0 <= 0 || C1 <= 0)
+    return fail(SGDCT_EINVAL, "bad upcat arguments");
+  return nn_ret(nn_upcat(low, skip, dst, B, H, W, C0, C1, (hipStream_t)stream), "upcat");
+}
+
+int sgdct_nn_upcat_back(const float* dv, float* glow, float* gskip, int B, int H, int W, int C0, int C1,
+                        void* stream) {
+  if (!dv || !glow || !gskip || !dims_ok(B, H, W) || H % 2 || W % 2 || C0 <= 0 || C1 <= 0)
+    return fail(SGDCT_EINVAL, "bad upcat_back arguments");
+  return nn_ret(nn_upcat_back(dv, glow, gskip, B, H, W, C0, C1, (hipStream_t)stream), "upcat_back");
+}
+
+int sgdct_nn_head(const void* d0, const float* w, const float* b, const float* x, float* y, long long M, int C,
+                  void* stream) {
+  if (!d0 || !w || !b || !x || !y || M <= 0 || C <= 0) return fail(SGDCT_EINVAL, "bad head arguments");
+  return nn_ret(nn_head(d0, w, b, x, y, M, C, (hipStream_t)stream), "head");
+}
+
+long long sgdct_nn_head_workspace(long long M, int C) {
+  if (M <= 0 || C <= 0) return fail(SGDCT_EINVAL, "bad head shape");
+  return (M + 4095) / 4096 * (long long)(C + 1);
+}
+
+int sgdct_nn_head_back(const float* y, const float* target, const float* w, const void* d0, float* dy, float* gd0,
+                       float* gw, float* workspace, long long workspace_floats, long long M, int C, void* stream) {
+  if (!y || !target || !w || !d0 || !dy || !gd0 || !gw || !workspace || M <= 0 || C <= 0 || C > 1023)
+    return fail(SGDCT_EINVAL, "bad head_back arguments");
+  if (workspace_floats < sgdct_nn_head_workspace(M, C)) return fail(SGDCT_EINVAL, "head workspace too small");
+  return nn_ret(nn_head_back(y, target, w, d0, dy, gd0, gw, workspace, M, C, (hipStream_t)stream), "head_back");
+}
+
+int sgdct_nn_relu_mask(const float* g, const void* act, void* out, long long n, void* stream) {
+  if (!g || !act || !out || n <= 0) return fail(SGDCT_EINVAL, "bad relu_mask arguments");
+  return nn_ret(nn_relu_mask(g, act, out, n, (hipStream_t)stream), "relu_mask");
+}
+
+int sgdct_nn_flip_weights(const void* w, void* wt, int cin, int cout, void* stream) {
+  if (!w || !wt || cin <= 0 || cout <= 0) return fail(SGDCT_EINVAL, "bad flip_weights arguments");
+  return nn_ret(nn_flip_weights(w, wt, cin, cout, (hipStream_t)stream), "flip_weights");
+}
+
+int sgdct_nn_adam(float* w32, void* w16, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+                  float eps, int step, void* stream) {
+  if (!w32 || !g || !m || !v || n <= 0 || step <= 0) return fail(SGDCT_EINVAL, "bad adam arguments");
+  return nn_ret(nn_adam(w32, w16, g, m, v, n, lr, b1, b2, eps, step, (hipStream_t)stream), "adam");
+}
+
+int sgdct_nn_fill(float* p, long long n, float value, void* stream) {
+  if (!p || n <= 0) return fail(SGDCT_EINVAL, "bad fill arguments");
+  return nn_ret(nn_fill(p, n, value, (hipStream_t)stream), "fill");
+}
+
+int sgdct_nn_to_bf16(const float* src, void* dst, long long n, void* stream) {
+  if (!src || !dst || n <= 0) return fail(SGDCT_EINVAL, "bad to_bf16 arguments");
+  return nn_ret(nn_to_bf16(src, dst, n, (hipStream_t)stream), "to_bf16");
+}
+
+int sgdct_axpy_batch(float* x, const float* g, const float* a, long long n, long long per, void* stream) {
+  if (!x || !g || !a || n <= 0 || per <= 0 || n % per) return fail(SGDCT_EINVAL, "bad axpy_batch arguments");
+  hipError_t e = launch_axpy_batch(x, g, a, (size_t)n, (size_t)per, (hipStream_t)stream);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, "axpy_batch launch");
 }
 
 int sgdct_tv_gradient(const float* x, float* g, int nx, int ny, int batch, float eps, void* stream) {

@@ -72,6 +72,19 @@ __global__ void axpy_kernel(float* __restrict__ x, const float* __restrict__ g, 
   x[k] = __builtin_fmaf(a, g[k], x[k]);
 }
 
+// x[k] += a[k / per] * g[k]  (one coefficient per batch entry, on the device)
+__global__ void axpy_batch_kernel(float* __restrict__ x, const float* __restrict__ g, const float* __restrict__ a,
+                                  size_t n, size_t per) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  x[k] = __builtin_fmaf(a[k / per], g[k], x[k]);
+}
+
+hipError_t launch_axpy_batch(float* x, const float* g, const float* a, size_t n, size_t per, hipStream_t s) {
+  axpy_batch_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(x, g, a, n, per);
+  return hipGetLastError();
+}
+
 hipError_t launch_tv_grad(const float* x, float* g, int nx, int ny, int batch, float eps, hipStream_t s) {
   tv_grad_kernel<<<dim3((nx + 15) / 16, (ny + 15) / 16, batch), dim3(256), 0, s>>>(x, g, nx, ny, eps);
   return hipGetLastError();

@@ -1,0 +1,521 @@
+// unet.hip -- gfx950 training and inference kernels of the warm-start U-Net.
+//
+// No reference counterpart (the reference's network code is a cuBLAS MLP,
+// network.cu / layers/linear.cu; its Layer::forward/back_prop pattern,
+// layers/layer.hh:15-16, is the closest precedent).  Activations are NHWC
+// bfloat16, gradients of activations FP32, weights bfloat16 W[cout][9 cin]
+// (tap-major: k = (3 ky + kx) cin + c) with an FP32 master copy.
+//
+//   conv3x3_kernel   3x3 zero-padded convolution as an implicit GEMM on the
+//                    matrix cores: v_mfma_f32_16x16x32_bf16, FP32 accumulate.
+//                    Block = 4 waves, each a 32-pixel x 32-channel tile
+//                    (2 x 2 MFMA tiles); operands are gathered straight from
+//                    HBM/L2 as 16-byte vectors (8 channels of one tap).
+//                    Bias + ReLU fused; stores bf16, or FP32 for the
+//                    input-gradient pass (which runs the same kernel on the
+//                    output gradient with the flipped, transposed weights).
+//   wgrad_kernel     dW[co][k] = sum_m dY[m][co] X[m + tap(k)][c(k)]: the
+//                    pixel sum is the GEMM's K; tiles of dY and X are staged
+//                    in LDS and read transposed; each block reduces a
+//                    contiguous pixel range into an FP32 partial, and a
+//                    second kernel sums the partials in a fixed order.
+//   small kernels    input padding, 2x2 average pool, nearest upsample +
+//                    concatenation and their adjoints, the 1x1 output head,
+//                    MSE gradient, ReLU-mask + bf16 conversion, weight
+//                    transpose, Adam.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sgdct {
+
+typedef __bf16 bf16;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct ConvArgs {
+  const bf16* in;      // [B][H][W][cin]
+  const bf16* w;       // [cout][9 cin]
+  const float* bias;   // [cout] or null
+  void* out;           // [B][H][W][cout], bf16 or FP32
+  int B, H, W, cin, cout;
+  int relu, out_f32;
+};
+
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) z[i] = (bf16)0.0f;
+  return z;
+}
+
+// One 32-pixel x 32-channel wave tile per (wm, wn); block tile WM*32 x WN*32.
+template <int WM, int WN>
+__global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
+  static_assert(WM * WN == 4, "four waves per block");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wm = wv % WM, wn = wv / WM;
+  const long long M = (long long)a.B * a.H * a.W;
+  const long long m_base = (long long)blockIdx.x * (WM * 32) + wm * 32;
+  const int n_base = blockIdx.y * (WN * 32) + wn * 32;
+  const int K = 9 * a.cin;
+  const int r16 = lane & 15, g = lane >> 4;
+
+  // pixel coordinates of the two A rows this lane loads
+  int py[2], px[2], pb[2];
+  bool pv[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const long long m = m_base + 16 * i + r16;
+    pv[i] = m < M;
+    const long long mm = pv[i] ? m : 0;
+    px[i] = (int)(mm % a.W);
+    const long long t = mm / a.W;
+    py[i] = (int)(t % a.H);
+    pb[i] = (int)(t / a.H);
+  }
+  int co[2];
+  bool cv[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    co[j] = n_base + 16 * j + r16;
+    cv[j] = co[j] < a.cout;
+  }
+
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int k = k0 + 8 * g;  // this lane's 8 consecutive k: one tap, 8 channels
+    const bool kv = k < K;
+    const int t = kv ? k / a.cin : 0, c = kv ? k - t * a.cin : 0;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    bf16x8 av[2], bv[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int yy = py[i] + dy, xx = px[i] + dx;
+      if (kv && pv[i] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+        av[i] = *reinterpret_cast<const bf16x8*>(a.in + (((size_t)pb[i] * a.H + yy) * a.W + xx) * a.cin + c);
+      else
+        av[i] = zero8();
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (kv && cv[j])
+        bv[j] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)co[j] * K + k);
+      else
+        bv[j] = zero8();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+  }
+
+  // D layout: lane holds rows 4 g + r (r = 0..3) of column r16
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    if (!cv[j]) continue;
+    const float bias = a.bias ? a.bias[co[j]] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long m = m_base + 16 * i + 4 * g + r;
+        if (m >= M) continue;
+        float v = acc[i][j][r] + bias;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.out_f32) static_cast<float*>(a.out)[m * a.cout + co[j]] = v;
+        else static_cast<bf16*>(a.out)[m * a.cout + co[j]] = (bf16)v;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient.  Block: 4 waves; output tile 64 (co) x 64 (k); each wave
+// one 16-row strip of co x all 64 k (4 MFMA tiles).  The block walks pixel
+// chunks of 32 [m0, m0 + chunk) and accumulates; result -> partial[blockIdx.z].
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+  const bf16* x;    // [B][H][W][cin]
+  const bf16* dy;   // [B][H][W][cout]
+  float* partial;   // [nsplit][cout][9 cin]
+  int B, H, W, cin, cout;
+  long long pix_per_split;
+};
+
+__global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
+  __shared__ bf16 s_dy[32][64 + 8];   // [m][co]
+  __shared__ bf16 s_x[32][64 + 8];    // [m][k]
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r16 = lane & 15, g = lane >> 4;
+  const int K = 9 * a.cin;
+  const int co0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const long long M = (long long)a.B * a.H * a.W;
+  const long long m_begin = (long long)blockIdx.z * a.pix_per_split;
+  const long long m_end = m_begin + a.pix_per_split < M ? m_begin + a.pix_per_split : M;
+
+  f32x4 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  // staging roles: thread -> (pixel row sm, 8-wide column group sc)
+  const int sm = threadIdx.x >> 3, sc = (threadIdx.x & 7) * 8;
+  for (long long m0 = m_begin; m0 < m_end; m0 += 32) {
+    {
+      const long long m = m0 + sm;
+      const bool mv = m < m_end;
+      // dY tile: channels co0 + sc .. +7
+      bf16x8 v = zero8();
+      if (mv && co0 + sc < a.cout) v = *reinterpret_cast<const bf16x8*>(a.dy + m * a.cout + co0 + sc);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s_dy[sm][sc + e] = v[e];
+      // X tile: k0 + sc .. +7 -> one tap, 8 channels, at pixel m shifted by the tap
+      bf16x8 xvv = zero8();
+      const int k = k0 + sc;
+      if (mv && k < K) {
+        const int t = k / a.cin, c = k - t * a.cin;
+        const int x = (int)(m % a.W);
+        const long long q = m / a.W;
+        const int y = (int)(q % a.H);
+        const int b = (int)(q / a.H);
+        const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+          xvv = *reinterpret_cast<const bf16x8*>(a.x + (((size_t)b * a.H + yy) * a.W + xx) * a.cin + c);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s_x[sm][sc + e] = xvv[e];
+    }
+    __syncthreads();
+    // A[co][m] = s_dy[m][co]: lane row co = 16 wv + r16, m = 8 g .. 8 g + 7
+    bf16x8 av;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) av[e] = s_dy[8 * g + e][16 * wv + r16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bf16x8 bv;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bv[e] = s_x[8 * g + e][16 * j + r16];
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float* out = a.partial + (size_t)blockIdx.z * a.cout * K;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = co0 + 16 * wv + 4 * g + r, k = k0 + 16 * j + r16;
+      if (co < a.cout && k < K) out[(size_t)co * K + k] = acc[j][r];
+    }
+}
+
+// out[i] = sum_s partial[s][i] (fixed order)
+__global__ void reduce_partials_kernel(const float* __restrict__ partial, float* __restrict__ out, long long n,
+                                       int nsplit) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int k = 0; k < nsplit; ++k) s += partial[(size_t)k * n + i];
+  out[i] = s;
+}
+
+// Column sums of an [M][C] bf16 or FP32 matrix into partial[split][C].
+template <typename T>
+__global__ void colsum_kernel(const T* __restrict__ a, float* __restrict__ partial, long long M, int C,
+                              long long rows_per_split) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const long long r0 = (long long)blockIdx.y * rows_per_split;
+  const long long r1 = r0 + rows_per_split < M ? r0 + rows_per_split : M;
+  float s = 0.0f;
+  for (long long r = r0; r < r1; ++r) s += (float)a[r * C + c];
+  partial[(size_t)blockIdx.y * C + c] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels.
+// ---------------------------------------------------------------------------
+__global__ void pad_input_kernel(const float* __restrict__ x, bf16* __restrict__ out8, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bf16x8 v = zero8();
+  v[0] = (bf16)x[i];
+  *reinterpret_cast<bf16x8*>(out8 + i * 8) = v;
+}
+
+// dst[b][y][x][c] = mean of src[b][2y+i][2x+j][c]
+__global__ void pool2_kernel(const bf16* __restrict__ src, bf16* __restrict__ dst, int B, int H, int W, int C) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)B * H * W * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  long long q = i / C;
+  const int x = (int)(q % W);
+  q /= W;
+  const int y = (int)(q % H);
+  const int b = (int)(q / H);
+  const size_t row = (size_t)2 * W * C;
+  const size_t o = (((size_t)b * 2 * H + 2 * y) * 2 * W + 2 * x) * C + c;
+  const float s = (float)src[o] + (float)src[o + C] + (float)src[o + row] + (float)src[o + row + C];
+  dst[i] = (bf16)(0.25f * s);
+}
+
+// g_src[b][2y+i][2x+j][c] += dv[b][y][x][c] / 4   (one thread per source element)
+__global__ void pool2_back_kernel(const float* __restrict__ dv, float* __restrict__ gsrc, int B, int H, int W, int C) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)B * 2 * H * 2 * W * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  long long q = i / C;
+  const int x = (int)(q % (2 * W));
+  q /= 2 * W;
+  const int y = (int)(q % (2 * H));
+  const int b = (int)(q / (2 * H));
+  gsrc[i] += 0.25f * dv[(((size_t)b * H + y / 2) * W + x / 2) * C + c];
+}
+
+// dst[b][y][x][0:C0] = low[b][y/2][x/2][:], dst[..][C0:C0+C1] = skip[b][y][x][:]
+__global__ void upcat_kernel(const bf16* __restrict__ low, const bf16* __restrict__ skip, bf16* __restrict__ dst,
+                             int B, int H, int W, int C0, int C1) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = C0 + C1;
+  const long long n = (long long)B * H * W * C;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  long long q = i / C;
+  const int x = (int)(q % W);
+  q /= W;
+  const int y = (int)(q % H);
+  const int b = (int)(q / H);
+  dst[i] = c < C0 ? low[(((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C0 + c]
+                  : skip[(((size_t)b * H + y) * W + x) * C1 + (c - C0)];
+}
+
+// g_low[b][y'][x'][c] = sum of the four children (c < C0); g_skip += rest
+__global__ void upcat_back_low_kernel(const float* __restrict__ dv, float* __restrict__ glow, int B, int H, int W,
+                                      int C0, int C1) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Hl = H / 2, Wl = W / 2, C = C0 + C1;
+  const long long n = (long long)B * Hl * Wl * C0;
+  if (i >= n) return;
+  const int c = (int)(i % C0);
+  long long q = i / C0;
+  const int x = (int)(q % Wl);
+  q /= Wl;
+  const int y = (int)(q % Hl);
+  const int b = (int)(q / Hl);
+  const size_t o = (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
+  const size_t row = (size_t)W * C;
+  glow[i] = dv[o] + dv[o + C] + dv[o + row] + dv[o + row + C];
+}
+
+__global__ void upcat_back_skip_kernel(const float* __restrict__ dv, float* __restrict__ gskip, int B, int H, int W,
+                                       int C0, int C1) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)B * H * W * C1;
+  if (i >= n) return;
+  const int c = (int)(i % C1);
+  const long long p = i / C1;
+  gskip[i] += dv[p * (C0 + C1) + C0 + c];
+}
+
+// y[m] = x[m] + sum_c d0[m][c] w[c] + b
+__global__ void head_kernel(const bf16* __restrict__ d0, const float* __restrict__ w, const float* __restrict__ b,
+                            const float* __restrict__ x, float* __restrict__ y, long long M, int C) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  float s = b[0];
+  for (int c = 0; c < C; ++c) s = __builtin_fmaf((float)d0[m * C + c], w[c], s);
+  y[m] = x[m] + s;
+}
+
+// dy = 2 (y - target) / n; g_d0[m][c] = dy[m] w[c]
+__global__ void head_back_kernel(const float* __restrict__ y, const float* __restrict__ target,
+                                 const float* __restrict__ w, float* __restrict__ dy, float* __restrict__ gd0,
+                                 long long M, int C, float scale) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const float d = scale * (y[m] - target[m]);
+  dy[m] = d;
+  for (int c = 0; c < C; ++c) gd0[m * C + c] = d * w[c];
+}
+
+// head weight gradient partials: part[s][c] = sum_m dy[m] d0[m][c], part[s][C] = sum_m dy[m]
+__global__ void head_wgrad_kernel(const float* __restrict__ dy, const bf16* __restrict__ d0, float* __restrict__ part,
+                                  long long M, int C, long long rows_per_split) {
+  const int c = threadIdx.x;  // blockDim.x = C + 1
+  const long long r0 = (long long)blockIdx.x * rows_per_split;
+  const long long r1 = r0 + rows_per_split < M ? r0 + rows_per_split : M;
+  float s = 0.0f;
+  for (long long r = r0; r < r1; ++r) s += dy[r] * (c < C ? (float)d0[r * C + c] : 1.0f);
+  part[(size_t)blockIdx.x * (C + 1) + c] = s;
+}
+
+// out = bf16(g * (act > 0))
+__global__ void relu_mask_kernel(const float* __restrict__ g, const bf16* __restrict__ act, bf16* __restrict__ out,
+                                 long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (bf16)((float)act[i] > 0.0f ? g[i] : 0.0f);
+}
+
+// wt[c][t][o] = w[o][8 - t][c]  (input-gradient weights)
+__global__ void flip_weights_kernel(const bf16* __restrict__ w, bf16* __restrict__ wt, int cin, int cout) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)cin * 9 * cout;
+  if (i >= n) return;
+  const int o = (int)(i % cout);
+  const long long q = i / cout;
+  const int t = (int)(q % 9);
+  const int c = (int)(q / 9);
+  wt[i] = w[((size_t)o * 9 + (8 - t)) * cin + c];
+}
+
+// Adam on an FP32 master tensor, with its bf16 copy (nullable)
+__global__ void adam_kernel(float* __restrict__ w32, bf16* __restrict__ w16, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, long long n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w32[i] - lr * (mi * c1) / (__builtin_sqrtf(vi * c2) + eps);
+  w32[i] = wi;
+  if (w16) w16[i] = (bf16)wi;
+}
+
+__global__ void fill_kernel(float* __restrict__ p, long long n, float value) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ src, bf16* __restrict__ dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (bf16)src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t nn_conv3x3(const void* in, const void* w, const float* bias, void* out, int B, int H, int W, int cin,
+                      int cout, int relu, int out_f32, hipStream_t s) {
+  ConvArgs a{static_cast<const bf16*>(in), static_cast<const bf16*>(w), bias, out, B, H, W, cin, cout, relu, out_f32};
+  const long long M = (long long)B * H * W;
+  if (cout % 64 == 0) {
+    conv3x3_kernel<2, 2><<<dim3(blocks_for(M, 64), cout / 64), 256, 0, s>>>(a);
+  } else {
+    conv3x3_kernel<4, 1><<<dim3(blocks_for(M, 128), (cout + 31) / 32), 256, 0, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+long long nn_wgrad_splits(long long M) {
+  // about 8192 pixels per block, at least one block
+  long long n = (M + 8191) / 8192;
+  return n < 1 ? 1 : n;
+}
+
+hipError_t nn_wgrad(const void* x, const void* dy, float* dw, float* partial, int B, int H, int W, int cin, int cout,
+                    hipStream_t s) {
+  const long long M = (long long)B * H * W;
+  const long long ns = nn_wgrad_splits(M);
+  const long long per = ((M + ns - 1) / ns + 31) / 32 * 32;
+  const int K = 9 * cin;
+  WgradArgs a{static_cast<const bf16*>(x), static_cast<const bf16*>(dy), partial, B, H, W, cin, cout, per};
+  wgrad_kernel<<<dim3((cout + 63) / 64, (K + 63) / 64, (unsigned)ns), 256, 0, s>>>(a);
+  const long long n = (long long)cout * K;
+  reduce_partials_kernel<<<blocks_for(n), 256, 0, s>>>(partial, dw, n, (int)ns);
+  return hipGetLastError();
+}
+
+hipError_t nn_colsum_bf16(const void* a, float* out, float* partial, long long M, int C, hipStream_t s) {
+  const long long ns = (M + 4095) / 4096;
+  colsum_kernel<bf16><<<dim3((C + 63) / 64, (unsigned)ns), 64, 0, s>>>(static_cast<const bf16*>(a), partial, M, C,
+                                                                      4096);
+  reduce_partials_kernel<<<blocks_for(C), 256, 0, s>>>(partial, out, C, (int)ns);
+  return hipGetLastError();
+}
+
+hipError_t nn_pad_input(const float* x, void* out8, long long n, hipStream_t s) {
+  pad_input_kernel<<<blocks_for(n), 256, 0, s>>>(x, static_cast<bf16*>(out8), n);
+  return hipGetLastError();
+}
+
+hipError_t nn_pool2(const void* src, void* dst, int B, int H, int W, int C, hipStream_t s) {
+  pool2_kernel<<<blocks_for((long long)B * H * W * C), 256, 0, s>>>(static_cast<const bf16*>(src),
+                                                                   static_cast<bf16*>(dst), B, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t nn_pool2_back(const float* dv, float* gsrc, int B, int H, int W, int C, hipStream_t s) {
+  pool2_back_kernel<<<blocks_for((long long)B * 4 * H * W * C), 256, 0, s>>>(dv, gsrc, B, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t nn_upcat(const void* low, const void* skip, void* dst, int B, int H, int W, int C0, int C1,
+                    hipStream_t s) {
+  upcat_kernel<<<blocks_for((long long)B * H * W * (C0 + C1)), 256, 0, s>>>(
+      static_cast<const bf16*>(low), static_cast<const bf16*>(skip), static_cast<bf16*>(dst), B, H, W, C0, C1);
+  return hipGetLastError();
+}
+
+hipError_t nn_upcat_back(const float* dv, float* glow, float* gskip, int B, int H, int W, int C0, int C1,
+                         hipStream_t s) {
+  upcat_back_low_kernel<<<blocks_for((long long)B * (H / 2) * (W / 2) * C0), 256, 0, s>>>(dv, glow, B, H, W, C0, C1);
+  upcat_back_skip_kernel<<<blocks_for((long long)B * H * W * C1), 256, 0, s>>>(dv, gskip, B, H, W, C0, C1);
+  return hipGetLastError();
+}
+
+hipError_t nn_head(const void* d0, const float* w, const float* b, const float* x, float* y, long long M, int C,
+                   hipStream_t s) {
+  head_kernel<<<blocks_for(M), 256, 0, s>>>(static_cast<const bf16*>(d0), w, b, x, y, M, C);
+  return hipGetLastError();
+}
+
+hipError_t nn_head_back(const float* y, const float* target, const float* w, const void* d0, float* dy, float* gd0,
+                        float* gw, float* partial, long long M, int C, hipStream_t s) {
+  head_back_kernel<<<blocks_for(M), 256, 0, s>>>(y, target, w, dy, gd0, M, C, 2.0f / (float)M);
+  const long long ns = (M + 4095) / 4096;
+  head_wgrad_kernel<<<(unsigned)ns, C + 1, 0, s>>>(dy, static_cast<const bf16*>(d0), partial, M, C, 4096);
+  reduce_partials_kernel<<<blocks_for(C + 1), 256, 0, s>>>(partial, gw, C + 1, (int)ns);
+  return hipGetLastError();
+}
+
+hipError_t nn_relu_mask(const float* g, const void* act, void* out, long long n, hipStream_t s) {
+  relu_mask_kernel<<<blocks_for(n), 256, 0, s>>>(g, static_cast<const bf16*>(act), static_cast<bf16*>(out), n);
+  return hipGetLastError();
+}
+
+hipError_t nn_flip_weights(const void* w, void* wt, int cin, int cout, hipStream_t s) {
+  flip_weights_kernel<<<blocks_for((long long)cin * 9 * cout), 256, 0, s>>>(static_cast<const bf16*>(w),
+                                                                           static_cast<bf16*>(wt), cin, cout);
+  return hipGetLastError();
+}
+
+hipError_t nn_adam(float* w32, void* w16, const float* g, float* m, float* v, long long n, float lr, float b1,
+                   float b2, float eps, int step, hipStream_t s) {
+  const float c1 = 1.0f / (1.0f - powf(b1, (float)step)), c2 = 1.0f / (1.0f - powf(b2, (float)step));
+  adam_kernel<<<blocks_for(n), 256, 0, s>>>(w32, static_cast<bf16*>(w16), g, m, v, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t nn_fill(float* p, long long n, float value, hipStream_t s) {
+  fill_kernel<<<blocks_for(n), 256, 0, s>>>(p, n, value);
+  return hipGetLastError();
+}
+
+hipError_t nn_to_bf16(const float* src, void* dst, long long n, hipStream_t s) {
+  to_bf16_kernel<<<blocks_for(n), 256, 0, s>>>(src, static_cast<bf16*>(dst), n);
+  return hipGetLastError();
+}
+
+}  // namespace sgdct

@@ -2,7 +2,8 @@
 layers/A_B0.cu:449-492, A_B1.cu:545-593) from the GPU against the oracle's
 restatement of the reference's matrix kernels (or_matrix_generation):
   * same COO triplets in the reference's order (view, bin, i, j), values to
-    <= 1e-5 relative Frobenius norm, counts within 0.5 %;
+    <= 1e-5 relative Frobenius norm, counts of entries above 1e-6 of the peak
+    within 0.2 %;
   * B1 rows are the footprint of the B1 projector: matrix @ x * dx * dy equals
     b1_forward_projection on the same views (the reference's own matrix kernel
     is one bin off, see the oracle);
@@ -54,7 +55,10 @@ def test_matrix_matches_oracle(cuda, oracle_exact, basis, geom):
     err = np.linalg.norm(M - Mo) / np.linalg.norm(Mo)
     print("%s %s: nnz gpu %d oracle %d, rel %.2e" % (basis, geom, n, len(vo), err))
     assert err <= 1e-5
-    assert abs(n - len(vo)) <= 0.005 * len(vo)
+    # entries above 1e-6 of the largest: FP32 footprints round the far tails
+    # (values ~1e-8 of the peak, which the FP64 oracle keeps) to zero
+    big, big_o = (M > 1e-6 * Mo.max()).sum(), (Mo > 1e-6 * Mo.max()).sum()
+    assert abs(int(big) - int(big_o)) <= 0.002 * big_o, (big, big_o)
 
 
 def test_b1_matrix_is_the_projector(cuda):
