@@ -153,3 +153,84 @@ def asd_pocs(P, b, niter=200, tol=None, x0=None, beta=1.0, beta_red=0.995, ng=20
         k += 1
     torch.cuda.synchronize()
     return AsdPocsResult(x, len([h for h in hist if h == h]), hist, time.perf_counter() - t0, fell_back)
+
+
+def _norms(t):
+    return torch.linalg.vector_norm(t.double(), dim=tuple(range(1, t.dim())))
+
+
+class AsdPocsBatchResult:
+    def __init__(self, x, iters, residuals, fell_back, seconds):
+        self.x, self.iters, self.residuals, self.fell_back, self.seconds = x, iters, residuals, fell_back, seconds
+
+
+def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, beta_red=0.995, ng=20, alpha=0.2,
+                   r_max=0.95, alpha_red=0.95, eps_data=0.0, tv_eps=1e-8, weights=None, check_every=10):
+    """ASD-POCS (Sidky & Pan 2008) on a batch of sinograms b (B, np, nu) with
+    per-image TV step control and stopping.  An image stops at the first
+    iteration whose entering residual ||A x - b|| / ||b|| is <= tol (its
+    iteration count is the number of data steps taken before that).  With
+    `fallback` (B, ny, nx), an image whose residual rises between two
+    iterations of its warm-started run restarts once from its fallback image.
+    The data step's relaxation beta follows one schedule for the batch."""
+    t0 = time.perf_counter()
+    L = _lib.lib()
+    b = b.contiguous().float()
+    nb = b.shape[0]
+    w = weights or SirtWeights(P)
+    dev = b.device
+    x = torch.zeros((nb,) + tuple(P.img_shape), device=dev) if x0 is None else \
+        x0.clone().contiguous().float().clamp_min(0)
+    bn = _norms(b)
+    active = torch.ones(nb, dtype=torch.bool, device=dev)
+    fell = torch.zeros(nb, dtype=torch.bool, device=dev)
+    iters = torch.full((nb,), niter, dtype=torch.int64, device=dev)
+    last_res = torch.full((nb,), float("nan"), dtype=torch.float64, device=dev)
+    prev_res = None
+    dtvg = torch.full((nb,), float("nan"), dtype=torch.float64, device=dev)
+    g = torch.empty_like(x)
+    coef = torch.empty(nb, device=dev)
+    per = x[0].numel()
+    for k in range(niter):
+        x_prev = x.clone()
+        ax = sirt_step(P, w, x, b, beta, positivity=True)
+        res = _norms(ax - b) / bn
+        last_res = torch.where(active, res, last_res)
+        if tol is not None:
+            done = active & (res <= tol)
+            iters = torch.where(done, torch.full_like(iters, k), iters)
+            active = active & ~done
+        if fallback is not None and prev_res is not None:
+            rise = active & ~fell & (res > prev_res)
+            x = torch.where(rise[:, None, None], fallback.float(), x)
+            fell = fell | rise
+            dtvg = torch.where(rise, torch.full_like(dtvg, float("nan")), dtvg)
+        prev_res = res
+        x = torch.where(active[:, None, None], x, x_prev).contiguous()
+        dp = _norms(x - x_prev)
+        dtvg = torch.where(torch.isnan(dtvg), alpha * dp, dtvg)
+        x_tv0 = x.clone()
+        for _ in range(ng):
+            tv_gradient(x, tv_eps, out=g)
+            gn = _norms(g)
+            c = torch.where(active & (gn > 0), -dtvg / gn.clamp_min(1e-300), torch.zeros_like(dtvg))
+            coef.copy_(c.float())
+            _lib.check(L.sgdct_axpy_batch(_vp(x), _vp(g), _vp(coef), x.numel(), per, _stream(x)), "axpy_batch")
+        dg = _norms(x - x_tv0)
+        dtvg = torch.where((dg > r_max * dp) & (res > eps_data), dtvg * alpha_red, dtvg)
+        beta *= beta_red
+        if tol is not None and (k + 1) % check_every == 0 and not bool(active.any()):
+            break
+    torch.cuda.synchronize()
+    return AsdPocsBatchResult(x, iters.cpu().numpy(), last_res.cpu().numpy(), fell.cpu().numpy(),
+                              time.perf_counter() - t0)
+
+
+def sirt_batch(P, b, niter, weights=None, x0=None, lam=1.0):
+    """niter SIRT iterations on a batch of sinograms (B, np, nu)."""
+    b = b.contiguous().float()
+    w = weights or SirtWeights(P)
+    x = torch.zeros((b.shape[0],) + tuple(P.img_shape), device=b.device) if x0 is None else x0.clone().float()
+    for _ in range(niter):
+        sirt_step(P, w, x, b, lam, positivity=True)
+    return x

@@ -1,0 +1,193 @@
+"""Learned warm start for ASD-POCS: data generation, training, evaluation.
+
+Workload (the sparse-view reconstruction of the learned-warm-start
+configuration): 512 x 512 images of 0.6 mm pixels, 90 views over 2 pi, 768
+bins of 0.8 mm, B1 basis, source-isocentre 500 mm, source-detector 1000 mm;
+random-ellipse phantoms (sgd_amd.phantom's generator) scaled by
+mu = 0.02 / mm, Poisson noise at 5e4 incident photons per ray,
+b = -ln(max(counts, 1) / N0) / mu.
+
+  pairs     input  = SIRT(5) of the noisy sinogram,
+            target = ASD-POCS(200) of the same sinogram (sgd_amd.solvers);
+  network   sgd_amd.unet.UNet (4 levels, 32 base channels, bf16 / FP32-acc.);
+  use       x0 = UNet(SIRT(5)) seeds ASD-POCS, which runs until the relative
+            data residual ||A x - b|| / ||b|| reaches the tolerance and
+            restarts from the SIRT(5) image if its residual rises (the plain
+            method stays the judge: the stopping rule is the same).
+
+`python -m sgd_amd.warmstart --help` runs the whole experiment and writes a
+JSON report.  No reference counterpart (the reference's optimizer/asd_pocs.cu
+is empty).
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+import torch
+
+from .phantom import ellipse_params
+from .solvers import SirtWeights, asd_pocs_batch, sirt_batch
+from .torch_ops import Projector
+from .unet import UNet
+
+GEOMETRY = dict(nx=512, ny=512, dx=0.6, dy=0.6, nu=768, du=0.8, lsd=1000.0, lso=500.0, views=90)
+MU = 0.02
+N0 = 5e4
+ASD = dict(alpha=0.2, r_max=0.95, ng=20, beta=1.0, beta_red=0.995, alpha_red=0.95)
+
+
+def projector(geom=GEOMETRY):
+    ang = 2 * np.pi * np.arange(geom["views"]) / geom["views"]
+    return Projector("B1", geom["nx"], geom["ny"], ang, geom["nu"], geom["dx"], geom["dy"], geom["du"], geom["lsd"],
+                     geom["lso"])
+
+
+def phantoms(seeds, nx, ny, dx, dy, device):
+    """Random-ellipse phantoms (the generator of sgd_amd.phantom), rasterised
+    on the device: (len(seeds), ny, nx) FP32."""
+    xs = (torch.arange(nx, device=device, dtype=torch.float64) - 0.5 * (nx - 1)) * dx
+    ys = (torch.arange(ny, device=device, dtype=torch.float64) - 0.5 * (ny - 1)) * dy
+    Y, X = torch.meshgrid(ys, xs, indexing="ij")
+    fov_half = 0.5 * min(nx * dx, ny * dy)
+    out = torch.zeros((len(seeds), ny, nx), device=device, dtype=torch.float64)
+    for i, s in enumerate(seeds):
+        for cx, cy, a, b, ang, d in ellipse_params(int(s), 10, fov_half):
+            c, sn = np.cos(ang), np.sin(ang)
+            xr = (X - cx) * c + (Y - cy) * sn
+            yr = -(X - cx) * sn + (Y - cy) * c
+            out[i] += d * (((xr / a) ** 2 + (yr / b) ** 2) <= 1.0)
+    return out.clamp_min_(0.0).float()
+
+
+def noisy_sinograms(P, x, seed, n0=N0, mu=MU):
+    """b = -ln(max(Poisson(n0 exp(-mu A x)), 1) / n0) / mu."""
+    y = P.forward(x).double()
+    g = torch.Generator(device=x.device).manual_seed(int(seed))
+    counts = torch.poisson(n0 * torch.exp(-mu * y), generator=g).clamp_min_(1.0)
+    return (-torch.log(counts / n0) / mu).float()
+
+
+def make_pairs(P, seeds, batch=64, sirt_iters=5, asd_iters=200, log=print):
+    """(inputs, targets, sinograms) for the phantoms of `seeds`."""
+    w = SirtWeights(P)
+    dev = P.device
+    ins, outs, sinos = [], [], []
+    t0 = time.perf_counter()
+    for k in range(0, len(seeds), batch):
+        sb = seeds[k:k + batch]
+        x = phantoms(sb, P.nx, P.ny, P.geom["dx"], P.geom["dy"], dev)
+        b = noisy_sinograms(P, x, seed=int(sb[0]) + 7919)
+        ins.append(sirt_batch(P, b, sirt_iters, weights=w))
+        outs.append(asd_pocs_batch(P, b, niter=asd_iters, weights=w, **ASD).x)
+        sinos.append(b)
+        log("pairs %d/%d  %.1fs" % (k + len(sb), len(seeds), time.perf_counter() - t0))
+    return torch.cat(ins), torch.cat(outs), torch.cat(sinos)
+
+
+def train(net, inputs, targets, epochs=3, batch=8, lr=1e-3, seed=0, log=print):
+    n = inputs.shape[0]
+    rng = np.random.default_rng(seed)
+    losses = []
+    steps = epochs * (n // batch)
+    t0 = time.perf_counter()
+    s = 0
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        run = 0.0
+        for k in range(0, n - batch + 1, batch):
+            idx = torch.as_tensor(order[k:k + batch], device=inputs.device)
+            lr_s = lr * 0.5 * (1 + np.cos(np.pi * s / steps))
+            y = net.train_step(inputs[idx], targets[idx], lr=lr_s)
+            if s % 50 == 0:
+                run = float(((y - targets[idx]) ** 2).mean())
+                losses.append((s, run))
+            s += 1
+        log("epoch %d  loss %.4e  %.1fs" % (ep, run, time.perf_counter() - t0))
+    return losses
+
+
+def evaluate(P, net, b, tol=1e-3, max_iters=3000, batch=64, sirt_iters=5, log=print):
+    """Plain ASD-POCS (from zero) vs warm-started (from UNet(SIRT(5))) to the
+    relative residual tol; per-image iterations and batch wall times."""
+    w = SirtWeights(P)
+    res = dict(plain_iters=[], warm_iters=[], plain_s=0.0, warm_s=0.0, fell_back=0, plain_res=[], warm_res=[])
+    for k in range(0, b.shape[0], batch):
+        bb = b[k:k + batch]
+        torch.cuda.synchronize()
+        r = asd_pocs_batch(P, bb, niter=max_iters, tol=tol, weights=w, **ASD)
+        res["plain_s"] += r.seconds
+        res["plain_iters"] += r.iters.tolist()
+        res["plain_res"] += r.residuals.tolist()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs = sirt_batch(P, bb, sirt_iters, weights=w)
+        x0 = net.forward(xs).clone()
+        r2 = asd_pocs_batch(P, bb, niter=max_iters, tol=tol, x0=x0, fallback=xs, weights=w, **ASD)
+        torch.cuda.synchronize()
+        res["warm_s"] += time.perf_counter() - t0
+        res["warm_iters"] += r2.iters.tolist()
+        res["warm_res"] += r2.residuals.tolist()
+        res["fell_back"] += int(r2.fell_back.sum())
+        log("eval %d/%d  plain %.0f its  warm %.0f its" % (k + bb.shape[0], b.shape[0], np.mean(r.iters),
+                                                          np.mean(r2.iters)))
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--pairs", type=int, default=4096)
+    ap.add_argument("--test", type=int, default=256)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--max-iters", type=int, default=3000)
+    ap.add_argument("--asd-iters", type=int, default=200)
+    ap.add_argument("--gen-batch", type=int, default=64)
+    ap.add_argument("--out", default="warmstart_report.json")
+    ap.add_argument("--save-weights", default=None)
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    P = projector()
+    t0 = time.perf_counter()
+    xin, xt, _ = make_pairs(P, list(range(a.pairs)), batch=a.gen_batch, asd_iters=a.asd_iters, log=log)
+    t_gen = time.perf_counter() - t0
+    net = UNet(4, 32, seed=0)
+    t0 = time.perf_counter()
+    losses = train(net, xin, xt, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    t_train = time.perf_counter() - t0
+    if a.save_weights:
+        np.savez(a.save_weights, **{k + "_W": v[0] for k, v in net.parameters_numpy().items()},
+                 **{k + "_b": v[1] for k, v in net.parameters_numpy().items()})
+    del xin, xt
+    seeds = list(range(10 ** 6, 10 ** 6 + a.test))
+    w = SirtWeights(P)
+    x = torch.cat([phantoms(seeds[k:k + 64], P.nx, P.ny, P.geom["dx"], P.geom["dy"], P.device)
+                   for k in range(0, len(seeds), 64)])
+    b = torch.cat([noisy_sinograms(P, x[k:k + 64], seed=seeds[k] + 7919) for k in range(0, len(seeds), 64)])
+    del x, w
+    ev = evaluate(P, net, b, tol=a.tol, max_iters=a.max_iters, log=log)
+    pi, wi = np.array(ev["plain_iters"]), np.array(ev["warm_iters"])
+    report = dict(workload=dict(GEOMETRY, mu=MU, n0=N0, asd=ASD, pairs=a.pairs, test=a.test, tol=a.tol,
+                                max_iters=a.max_iters, asd_iters_target=a.asd_iters, epochs=a.epochs),
+                  seconds=dict(generate=t_gen, train=t_train, plain=ev["plain_s"], warm=ev["warm_s"]),
+                  iterations=dict(plain_mean=float(pi.mean()), warm_mean=float(wi.mean()),
+                                  plain_median=float(np.median(pi)), warm_median=float(np.median(wi)),
+                                  plain_reached=int((pi < a.max_iters).sum()),
+                                  warm_reached=int((wi < a.max_iters).sum())),
+                  speedup_iterations=float(pi.mean() / max(wi.mean(), 1e-9)),
+                  speedup_time=float(ev["plain_s"] / max(ev["warm_s"], 1e-9)),
+                  fell_back=ev["fell_back"], train_loss=losses[-5:])
+    with open(a.out, "w") as f:
+        json.dump(report, f, indent=1)
+    log(json.dumps({k: report[k] for k in ("seconds", "iterations", "speedup_iterations", "speedup_time",
+                                           "fell_back")}))
+    return report
+
+
+if __name__ == "__main__":
+    main()
