@@ -102,6 +102,22 @@ sgdct_projector *b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, doub
 unsigned long long b1_3d_forward_projection(double *b, double *x, sgdct_projector *h);
 int b1_3d_backward_projection(double *b, double *x, sgdct_projector *h);
 
+/* layers/SF_projector.hh / SF_projector.cu:81-400 (class SF; the reference has
+ * no C entry points for it): cone-beam separable footprints, trap_v = 0 SF-TR
+ * (rectangle in v), 1 SF-TT (trapezoid in v).  Same layouts as b1_3d_*;
+ * detector cells centred on integer coordinates (SF_projector.cu:209-216).
+ * Fixed against the reference: the value carries the voxel and the amplitude
+ * lsd^2 dx dy dz / (du dv r^2) (its forward adds only the footprint product,
+ * :230-231, and :364 has dy dz), SF-TT's v area is 2 / ((v3-v0) + (v2-v1))
+ * (:195 misplaces the parenthesis), r is measured from the voxel centre (:103
+ * uses nx / 2 in integers), cells off the detector are dropped (:228 wraps
+ * into the next row), and backward is the transpose of forward (:394 scales
+ * by lsd dx / np).  Both return SGDCT_OK or a negative code. */
+sgdct_projector *sf3d_init(int nx, int ny, int nz, int np, int nu, int nv, double dx, double dy, double dz,
+                           double du, double dv, double lsd, double lso, const double *angles, int trap_v);
+int sf3d_forward_projection(double *b, double *x, sgdct_projector *h);
+int sf3d_backward_projection(double *b, double *x, sgdct_projector *h);
+
 /* ---- additions ------------------------------------------------------------ */
 
 /* Release a handle from any *_init (the reference leaks them). */
@@ -121,7 +137,8 @@ int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int ba
 int sgdct_shape(const sgdct_projector *h, int *out);
 
 /* Shape query for any handle: out[0..5] = nx, ny, nz, np, nu, nv (nz = nv = 1
- * for the fan-beam bases); returns the basis id (5 = cone-beam B1). */
+ * for the fan-beam bases); returns the basis id (5 = cone-beam B1, 6 = cone-beam
+ * separable footprints). */
 int sgdct_shape3(const sgdct_projector *h, int *out);
 
 /* Footprint evaluations of one A under the reference's corner rule (the

@@ -85,6 +85,11 @@ class Oracle:
                 f.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int]
             lib.or_b1_3d_cell_exact.restype = ctypes.c_double
             lib.or_b1_3d_cell_exact.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7
+            for name in ("or_sf3d_forward", "or_sf3d_backward"):
+                f = getattr(lib, name)
+                f.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int]
+            lib.or_sf3d_cell_exact.restype = ctypes.c_double
+            lib.or_sf3d_cell_exact.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7
             Oracle._libs[path] = lib
         self.lib = Oracle._libs[path]
         self.mode = mode
@@ -262,6 +267,24 @@ class Geometry3:
 
     def cell_exact(self, p, u, j, ix, iy, iz, panels=8):
         return self.lib.or_b1_3d_cell_exact(self.h, p, u, j, ix, iy, iz, panels)
+
+    # separable footprints (layers/SF_projector.cu), trap_v: SF-TT, else SF-TR
+    def sf_forward(self, vol, trap_v=False, views=None):
+        vol = np.ascontiguousarray(vol, dtype=np.float64).reshape(self.nz, self.ny, self.nx)
+        out = np.zeros((self.np, self.nv, self.nu))
+        p0, p1 = self._views(views)
+        self.lib.or_sf3d_forward(self.h, int(bool(trap_v)), _ptr(vol), _ptr(out), p0, p1)
+        return out
+
+    def sf_backward(self, proj, trap_v=False, views=None):
+        proj = np.ascontiguousarray(proj, dtype=np.float64).reshape(self.np, self.nv, self.nu)
+        out = np.zeros((self.nz, self.ny, self.nx))
+        p0, p1 = self._views(views)
+        self.lib.or_sf3d_backward(self.h, int(bool(trap_v)), _ptr(proj), _ptr(out), p0, p1)
+        return out
+
+    def sf_cell_exact(self, p, u, j, ix, iy, iz, panels=8):
+        return self.lib.or_sf3d_cell_exact(self.h, p, u, j, ix, iy, iz, panels)
 
 
 def matrix_generation(oracle, basis, np_, nu, nx, ny, dx, dy, du, lsd, lso, buffer_size=None, fixed=True):

@@ -1441,4 +1441,181 @@ int or_matrix_generation(int basis, int np, int nu, int nx, int ny, double dx, d
     return (int)idx;
 }
 
+/* ------------------------------------------------------------------------- */
+/* 3-D separable footprints, layers/SF_projector.cu:81-234: the voxel's u     */
+/* trapezoid (corners ix +- 1/2, iy +- 1/2) times a rectangle (SF-TR: the     */
+/* centre column at iz +- 1/2) or a trapezoid (SF-TT: x index +- 1/2 at iy,   */
+/* iz +- 1/2) in v, on detector cells centred on integer coordinates          */
+/* ([i - 1/2, i + 1/2], SF_projector.cu:209-216), amplitude                   */
+/* lsd^2 dx dy dz / (du dv) / r^2 times the inverse areas of the two          */
+/* footprints (SF_projector.cu:193-198, 364).  Fixed against the reference:   */
+/* the voxel volume dx dy dz (:364 has dy dz, the volume only when dx = 1;    */
+/* against exact chord integrals its mass is off by 1/dx); the value          */
+/* carries the voxel and the amplitude (its forward adds only the footprint   */
+/* product f2, :230-231, leaving C unused), SF-TT's v area is                 */
+/* 2 / ((v3 - v0) + (v2 - v1)) (:195 reads 2 / (v3 - v0) + (v2 - v1)), r is   */
+/* measured from the voxel centre ((nx - 1) / 2 offsets; :103 uses nx / 2 in  */
+/* integers), and cells outside the detector are dropped (:228 wraps u past   */
+/* a row end into the next row).  Same volume / projection layouts as         */
+/* or_b1_3d_*; the transpose is or_sf3d_backward.                             */
+/* ------------------------------------------------------------------------- */
+static int sf3d_voxel(const ogeo3 *t, int p, int ix, int iy, int iz, int trap_v, fpt *us, fpt *vs, int *u0, int *u1,
+                      int *v0, int *v1, fpt *amp) {
+    const ogeo *g = t->g;
+    const fpt *pm = g->pmi + 12 * p, *pv = t->pmv + 4 * p;
+    int nu = g->nu, nv = t->nv, dummy0, dummy1;
+    corner_range(pm, ix, iy, (fpt)0.5f, nu, us, &dummy0, &dummy1); /* sorted corner projections */
+    fpt z1 = (fpt)iz - (fpt)0.5f, z2 = (fpt)iz + (fpt)0.5f;
+    fpt av;
+    if (!trap_v) {
+        fpt num = pv[0] * (fpt)ix + pv[1] * (fpt)iy + pv[3];
+        fpt den = pm[8] * (fpt)ix + pm[9] * (fpt)iy + pm[11];
+        vs[0] = (num + pv[2] * z1) / den;
+        vs[1] = (num + pv[2] * z2) / den;
+        if (ceil((double)vs[0] - 0.5) >= nv) return 0;
+        av = (fpt)1.0f / (vs[1] - vs[0]);
+        *v0 = (int)ceil((double)vs[0] - 0.5);
+        *v1 = (int)ceil((double)vs[1] - 0.5);
+    } else {
+        fpt sx1 = (fpt)ix - (fpt)0.5f, sx2 = (fpt)ix + (fpt)0.5f, fy = (fpt)iy;
+        fpt n1 = pv[0] * sx1 + pv[1] * fy + pv[3], n2 = pv[0] * sx2 + pv[1] * fy + pv[3];
+        fpt d1 = pm[8] * sx1 + pm[9] * fy + pm[11], d2 = pm[8] * sx2 + pm[9] * fy + pm[11];
+        vs[0] = (n1 + pv[2] * z1) / d1;
+        vs[1] = (n1 + pv[2] * z2) / d1;
+        vs[2] = (n2 + pv[2] * z1) / d2;
+        vs[3] = (n2 + pv[2] * z2) / d2;
+#define OR_SORT2(a, b) do { if ((a) > (b)) { fpt t_ = (b); (b) = (a); (a) = t_; } } while (0)
+        OR_SORT2(vs[0], vs[1]);
+        OR_SORT2(vs[2], vs[3]);
+        OR_SORT2(vs[0], vs[2]);
+        OR_SORT2(vs[1], vs[3]);
+        OR_SORT2(vs[1], vs[2]);
+#undef OR_SORT2
+        av = (fpt)2.0f / ((vs[3] - vs[0]) + (vs[2] - vs[1]));
+        *v0 = (int)ceil((double)vs[0] - 0.5);
+        *v1 = (int)ceil((double)vs[3] - 0.5);
+    }
+    *u0 = (int)ceil((double)us[0] - 0.5);
+    *u1 = (int)ceil((double)us[3] - 0.5);
+    if (*v1 < 0 || *v0 >= nv || *u1 < 0 || *u0 >= nu) return -1;
+    *u0 = imax(*u0, 0); *u1 = imin(*u1, nu - 1);
+    *v0 = imax(*v0, 0); *v1 = imin(*v1, nv - 1);
+    double dx = g->dx, dy = g->dy, dz = t->dz;
+    double ox = (ix - 0.5 * (g->nx - 1)) * dx - g->src[2 * p], oy = (iy - 0.5 * (g->ny - 1)) * dy - g->src[2 * p + 1];
+    double oz = (iz - 0.5 * (t->nz - 1)) * dz;
+    double lsd = g->lsd;
+    double fac = lsd * lsd * dx * dy * dz / (g->du * t->dv);
+    double w = 1.0 / (ox * ox + oy * oy + oz * oz);
+    *amp = (fpt)(w * fac * (double)((fpt)2.0f / ((us[3] - us[0]) + (us[2] - us[1]))) * (double)av);
+    return 1;
+}
+
+static fpt sf3d_row_weight(int trap_v, const fpt *vs, int j) {
+    fpt s1 = (fpt)j - (fpt)0.5f, s2 = (fpt)j + (fpt)0.5f;
+    if (trap_v) return sf_gamma(s1, s2, vs);
+    fpt a = s2 < vs[1] ? s2 : vs[1], b = s1 > vs[0] ? s1 : vs[0];
+    return a - b > 0 ? a - b : (fpt)0.0f;
+}
+
+void or_sf3d_forward(const void *h, int trap_v, const double *vol, double *proj, int p0, int p1) {
+    const ogeo3 *t = (const ogeo3 *)h;
+    const ogeo *g = t->g;
+    int nx = g->nx, ny = g->ny, nz = t->nz, nu = g->nu, nv = t->nv;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = p0; p < p1; ++p) {
+        double *pj = proj + (size_t)p * nv * nu;
+        memset(pj, 0, sizeof(double) * (size_t)nv * nu);
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix) {
+                    double x = vol[((size_t)iz * ny + iy) * nx + ix];
+                    if (x == 0.0) continue;
+                    fpt us[4], vs[4], amp;
+                    int u0, u1, v0, v1;
+                    if (sf3d_voxel(t, p, ix, iy, iz, trap_v, us, vs, &u0, &u1, &v0, &v1, &amp) <= 0) continue;
+                    for (int u = u0; u <= u1; ++u) {
+                        fpt gu = sf_gamma((fpt)u - (fpt)0.5f, (fpt)u + (fpt)0.5f, us);
+                        for (int j = v0; j <= v1; ++j) {
+                            double f = (double)amp * (double)gu * (double)sf3d_row_weight(trap_v, vs, j);
+                            if (f == f) pj[(size_t)j * nu + u] += x * f;
+                        }
+                    }
+                }
+    }
+}
+
+void or_sf3d_backward(const void *h, int trap_v, const double *proj, double *vol, int p0, int p1) {
+    const ogeo3 *t = (const ogeo3 *)h;
+    const ogeo *g = t->g;
+    int nx = g->nx, ny = g->ny, nz = t->nz, nu = g->nu, nv = t->nv;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int col = 0; col < nx * ny; ++col) {
+        int ix = col % nx, iy = col / nx;
+        for (int iz = 0; iz < nz; ++iz) {
+            double acc = 0.0;
+            for (int p = p0; p < p1; ++p) {
+                const double *pj = proj + (size_t)p * nv * nu;
+                fpt us[4], vs[4], amp;
+                int u0, u1, v0, v1;
+                if (sf3d_voxel(t, p, ix, iy, iz, trap_v, us, vs, &u0, &u1, &v0, &v1, &amp) <= 0) continue;
+                for (int u = u0; u <= u1; ++u) {
+                    fpt gu = sf_gamma((fpt)u - (fpt)0.5f, (fpt)u + (fpt)0.5f, us);
+                    for (int j = v0; j <= v1; ++j) {
+                        double f = (double)amp * (double)gu * (double)sf3d_row_weight(trap_v, vs, j);
+                        if (f == f) acc += f * pj[(size_t)j * nu + u];
+                    }
+                }
+            }
+            vol[((size_t)iz * ny + iy) * nx + ix] = acc;
+        }
+    }
+}
+
+/* chord length of the line P + s e (unit e) through the box [lo, hi] */
+static double box_chord3(const double *P, const double *e, const double *lo, const double *hi) {
+    double a = -1e300, b = 1e300;
+    for (int k = 0; k < 3; ++k) {
+        if (fabs(e[k]) < 1e-300) {
+            if (P[k] < lo[k] || P[k] > hi[k]) return 0.0;
+            continue;
+        }
+        double t0 = (lo[k] - P[k]) / e[k], t1 = (hi[k] - P[k]) / e[k];
+        if (t0 > t1) { double tt = t0; t0 = t1; t1 = tt; }
+        if (t0 > a) a = t0;
+        if (t1 < b) b = t1;
+    }
+    return b > a ? b - a : 0.0;
+}
+
+/* FP64 mean over SF detector cell (u, j) = [u - 1/2, u + 1/2] x [j - 1/2,
+ * j + 1/2] of the chord length of source -> detector-point rays through voxel
+ * (ix, iy, iz) (composite Gauss-Legendre, panels x panels of 8 x 8). */
+double or_sf3d_cell_exact(const void *h, int p, int u, int j, int ix, int iy, int iz, int panels) {
+    const ogeo3 *t = (const ogeo3 *)h;
+    const ogeo *g = t->g;
+    double c[3] = {(ix - 0.5 * (g->nx - 1)) * g->dx, (iy - 0.5 * (g->ny - 1)) * g->dy,
+                   (iz - 0.5 * (t->nz - 1)) * t->dz};
+    double lo[3] = {c[0] - 0.5 * g->dx, c[1] - 0.5 * g->dy, c[2] - 0.5 * t->dz};
+    double hi[3] = {c[0] + 0.5 * g->dx, c[1] + 0.5 * g->dy, c[2] + 0.5 * t->dz};
+    double P[3] = {g->src[2 * p], g->src[2 * p + 1], 0.0};
+    double pux = g->puv[2 * p], puy = g->puv[2 * p + 1];
+    double total = 0.0;
+    for (int a = 0; a < panels; ++a)
+        for (int qa = 0; qa < 8; ++qa) {
+            double s = (a + 0.5 + 0.5 * GL8x[qa]) / panels - 0.5, ws = GL8w[qa] * 0.5 / panels;
+            double qx = g->dtv[2 * p] + pux * (u + s - 0.5 * g->nu);
+            double qy = g->dtv[2 * p + 1] + puy * (u + s - 0.5 * g->nu);
+            for (int b = 0; b < panels; ++b)
+                for (int qb = 0; qb < 8; ++qb) {
+                    double tt = (b + 0.5 + 0.5 * GL8x[qb]) / panels - 0.5, wt = GL8w[qb] * 0.5 / panels;
+                    double qz = t->dv * (j + tt - 0.5 * t->nv);
+                    double e[3] = {qx - P[0], qy - P[1], qz - P[2]};
+                    double nn = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+                    e[0] /= nn; e[1] /= nn; e[2] /= nn;
+                    total += ws * wt * box_chord3(P, e, lo, hi);
+                }
+        }
+    return total;
+}
+
 int or_precision_bits(void) { return (int)(sizeof(fpt) * 8); }

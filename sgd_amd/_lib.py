@@ -71,6 +71,12 @@ def lib():
     L.b1_3d_forward_projection.restype = ctypes.c_ulonglong
     L.b1_3d_backward_projection.argtypes = [_dp, _dp, _h]
     L.b1_3d_backward_projection.restype = ctypes.c_int
+    L.sf3d_init.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double] * 7 + [_dp, ctypes.c_int]
+    L.sf3d_init.restype = _h
+    for name in ("sf3d_forward_projection", "sf3d_backward_projection"):
+        f = getattr(L, name)
+        f.argtypes = [_dp, _dp, _h]
+        f.restype = ctypes.c_int
     _v, _i, _ll, _f = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
     sigs = {
         "sgdct_nn_conv3x3": [_v, _v, _v, _v] + [_i] * 7 + [_v],

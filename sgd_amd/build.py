@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsgdct.so")
-SOURCES = ["capi.cpp", "proj2d.hip", "sf2d.hip", "hex2d.hip", "solver.hip", "b1_3d.hip", "batch.hip", "matgen.hip", "unet.hip"]
+SOURCES = ["capi.cpp", "proj2d.hip", "sf2d.hip", "hex2d.hip", "solver.hip", "cone3d.hip", "batch.hip", "matgen.hip", "unet.hip"]
 ARCH = os.environ.get("SGDCT_OFFLOAD_ARCH", "gfx950")
 
 

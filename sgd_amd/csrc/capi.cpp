@@ -37,10 +37,10 @@ hipError_t launch_update(float* x, const float* c, const float* v, float lam, in
                          hipStream_t s);
 hipError_t launch_axpy(float* x, const float* g, float a, size_t n, hipStream_t s);
 hipError_t launch_transpose(const float* in, float* out, int rows, int cols, int batch, hipStream_t s);
-hipError_t launch_b1_3d_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y,
-                                      float* proj, hipStream_t s);
-hipError_t launch_b1_3d_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb,
-                                       float* xT, int accumulate, hipStream_t s);
+hipError_t launch_cone_forward_chunk(int model, const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y,
+                                     float* proj, hipStream_t s);
+hipError_t launch_cone_backward_chunk(int model, const Cone3DArgs& a, int p0, int nview, const float* projT,
+                                      float* Yb, float* xT, int accumulate, hipStream_t s);
 constexpr int kMaxNz3D = 512, kMaxNv3D = 512;
 hipError_t launch_matgen(int basis, const Proj2DArgs& g, bool write, unsigned int* counts, const long long* offsets,
                          double* out, hipStream_t s);
@@ -129,6 +129,7 @@ struct sgdct_projector {
   // cone beam (BASIS_B1_3D): volume nz slices, detector nv rows
   int nz = 1, nv = 1;
   double dz = 0, dv = 0;
+  int cone_model = 0;         // 0 B1, 1 SF-TR, 2 SF-TT
   int chunk = 0;              // views per pass of the staged kernels
   float* d_xT = nullptr;      // [ncol][nz] column-major volume
   float* d_Y = nullptr;       // [chunk][ncol][nv] per-column row profiles
@@ -235,8 +236,11 @@ void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, vo
 
 bool is_hex(int basis) { return basis == BASIS_H0 || basis == BASIS_HL; }
 
+// ushift moves the bin edges by ushift bins: 0.5 gives cells centred on
+// integer detector coordinates (layers/SF_projector.cu:209-216).
 sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, double dy, double du,
-                        double lsd, double lso, const double* angles, const double* drho, double dt = 0.0) {
+                        double lsd, double lso, const double* angles, const double* drho, double dt = 0.0,
+                        double ushift = 0.0) {
   if (nx <= 0 || ny <= 0 || np <= 0 || nu <= 0 || !(dx > 0) || !(dy > 0) || !(du > 0) ||
       !(lsd > 0) || !(lso > 0) || !angles) {
     fail(SGDCT_EINVAL, "invalid geometry arguments");
@@ -263,8 +267,8 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
     v.py = src[1];
     v.pux = pu[0];
     v.puy = pu[1];
-    v.e0x = dtv[0] - 0.5 * nu * pu[0] - src[0];
-    v.e0y = dtv[1] - 0.5 * nu * pu[1] - src[1];
+    v.e0x = dtv[0] - (0.5 * nu + ushift) * pu[0] - src[0];
+    v.e0y = dtv[1] - (0.5 * nu + ushift) * pu[1] - src[1];
     v.w0x = -cxo - src[0];
     v.w0y = -cyo - src[1];
     v.gx0 = (float)(-v.e0y * dx);
@@ -331,8 +335,11 @@ Cone3DArgs cone_args(const sgdct_projector* h) {
   a.cvol = (float)(h->dx * h->dy * h->dz);
   a.dz = (float)h->dz;
   a.du = h->du;
+  a.fac = (float)(h->lsd * h->lsd * h->dx * h->dy * h->dz / (h->du * h->dv));
   return a;
 }
+
+bool is_cone(int basis) { return basis == BASIS_B1_3D || basis == BASIS_SF_3D; }
 
 int ensure_cone_workspace(sgdct_projector* h) {
   if (h->d_xT) return SGDCT_OK;
@@ -359,7 +366,7 @@ int cone_forward(sgdct_projector* h, const float* vol, float* proj, int batch, h
     float* y = proj + (size_t)b * h->np * h->nv * h->nu;
     hipError_t e = launch_transpose(x, h->d_xT, h->nz, ncol, 1, s);
     for (int p0 = 0; p0 < h->np && e == hipSuccess; p0 += h->chunk)
-      e = launch_b1_3d_forward_chunk(a, p0, std::min(h->chunk, h->np - p0), h->d_xT, h->d_Y, y, s);
+      e = launch_cone_forward_chunk(h->cone_model, a, p0, std::min(h->chunk, h->np - p0), h->d_xT, h->d_Y, y, s);
     if (e != hipSuccess) return hip_fail(e, "cone-beam forward launch");
   }
   return SGDCT_OK;
@@ -378,7 +385,7 @@ int cone_backward(sgdct_projector* h, const float* proj, float* vol, int batch, 
       const int nview = std::min(h->chunk, h->np - p0);
       e = launch_transpose(y + (size_t)p0 * h->nv * h->nu, h->d_projT, h->nv, h->nu, nview, s);
       if (e == hipSuccess)
-        e = launch_b1_3d_backward_chunk(a, p0, nview, h->d_projT, h->d_Y, h->d_xT, p0 > 0, s);
+        e = launch_cone_backward_chunk(h->cone_model, a, p0, nview, h->d_projT, h->d_Y, h->d_xT, p0 > 0, s);
     }
     if (e == hipSuccess) e = launch_transpose(h->d_xT, x, ncol, h->nz, 1, s);
     if (e != hipSuccess) return hip_fail(e, "cone-beam backward launch");
@@ -387,7 +394,7 @@ int cone_backward(sgdct_projector* h, const float* proj, float* vol, int batch, 
 }
 
 int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
-  if (h->basis == BASIS_B1_3D) return cone_forward(h, img, sino, batch, s);
+  if (is_cone(h->basis)) return cone_forward(h, img, sino, batch, s);
   hipError_t e;
   if (is_hex(h->basis)) e = launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, s);
   else if (h->basis == BASIS_SF) e = launch_sf_forward(h->args, img, sino, batch, s);
@@ -396,7 +403,7 @@ int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hi
 }
 
 int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
-  if (h->basis == BASIS_B1_3D) return cone_backward(h, sino, img, batch, s);
+  if (is_cone(h->basis)) return cone_backward(h, sino, img, batch, s);
   hipError_t e;
   if (is_hex(h->basis)) e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
   else if (h->basis == BASIS_SF) e = launch_sf_backward(h->args, sino, img, batch, s);
@@ -440,6 +447,7 @@ long long cone_eval_count(const sgdct_projector* h, const std::vector<ViewGeo>& 
 
 long long eval_count(sgdct_projector* h) {
   if (h->evals >= 0) return h->evals;
+  if (h->basis == BASIS_SF_3D) return fail(SGDCT_EUNSUP, "no evaluation count for the separable-footprint projector");
   if (h->basis == BASIS_B1_3D) {
     std::vector<ViewGeo> views(h->np);
     hipError_t e = hipMemcpy(views.data(), h->d_views, sizeof(ViewGeo) * h->np, hipMemcpyDeviceToHost);
@@ -462,7 +470,7 @@ long long eval_count(sgdct_projector* h) {
 // Host-array forward: x[ix*ny+iy] (FP64) -> b[p*nu+u] (FP64).
 int host_forward(double* b, double* x, sgdct_projector* h) {
   if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
-  if (h->basis == BASIS_B1_3D) return fail(SGDCT_EINVAL, "cone-beam handle: use b1_3d_forward_projection");
+  if (is_cone(h->basis)) return fail(SGDCT_EINVAL, "cone-beam handle: use the *_3d entry points");
   try {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "set device");
@@ -490,7 +498,7 @@ int host_forward(double* b, double* x, sgdct_projector* h) {
 // Host-array back-projection: b[p*nu+u] -> x[ix*ny+iy] (overwritten).
 int host_backward(double* b, double* x, sgdct_projector* h) {
   if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
-  if (h->basis == BASIS_B1_3D) return fail(SGDCT_EINVAL, "cone-beam handle: use b1_3d_backward_projection");
+  if (is_cone(h->basis)) return fail(SGDCT_EINVAL, "cone-beam handle: use the *_3d entry points");
   try {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "set device");
@@ -515,9 +523,9 @@ int host_backward(double* b, double* x, sgdct_projector* h) {
 
 // Cone-beam host arrays: x copied raw ([iz][iy][ix], A_B1_3D.cu:716),
 // b = [p][v][u] (A_B1_3D.cu:731).
-long long cone_host_forward(double* b, double* x, sgdct_projector* h) {
+long long cone_host_forward(double* b, double* x, sgdct_projector* h, int basis) {
   if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
-  if (h->basis != BASIS_B1_3D) return fail(SGDCT_EINVAL, "not a cone-beam handle");
+  if (h->basis != basis) return fail(SGDCT_EINVAL, "handle of another projector");
   try {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "set device");
@@ -532,15 +540,15 @@ long long cone_host_forward(double* b, double* x, sgdct_projector* h) {
     e = hipMemcpy(h->h_sino, h->d_sino, ns * sizeof(float), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "download projections");
     for (size_t i = 0; i < ns; ++i) b[i] = (double)h->h_sino[i];
-    return eval_count(h);
+    return basis == BASIS_B1_3D ? eval_count(h) : SGDCT_OK;
   } catch (...) {
     return fail(SGDCT_EDEVICE, "unexpected exception in forward");
   }
 }
 
-int cone_host_backward(double* b, double* x, sgdct_projector* h) {
+int cone_host_backward(double* b, double* x, sgdct_projector* h, int basis) {
   if (!h || !b || !x) return fail(SGDCT_EINVAL, "null argument");
-  if (h->basis != BASIS_B1_3D) return fail(SGDCT_EINVAL, "not a cone-beam handle");
+  if (h->basis != basis) return fail(SGDCT_EINVAL, "handle of another projector");
   try {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "set device");
@@ -743,10 +751,43 @@ sgdct_projector* b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, doub
 }
 
 unsigned long long b1_3d_forward_projection(double* b, double* x, sgdct_projector* h) {
-  return (unsigned long long)cone_host_forward(b, x, h);
+  return (unsigned long long)cone_host_forward(b, x, h, BASIS_B1_3D);
 }
 
-int b1_3d_backward_projection(double* b, double* x, sgdct_projector* h) { return cone_host_backward(b, x, h); }
+int b1_3d_backward_projection(double* b, double* x, sgdct_projector* h) {
+  return cone_host_backward(b, x, h, BASIS_B1_3D);
+}
+
+// layers/SF_projector.hh / SF_projector.cu:359-400 (class SF, no C entry
+// points in the reference); trap_v selects SF-TT, else SF-TR.
+sgdct_projector* sf3d_init(int nx, int ny, int nz, int np, int nu, int nv, double dx, double dy, double dz, double du,
+                           double dv, double lsd, double lso, const double* angles, int trap_v) {
+  if (nz <= 0 || nv <= 0 || !(dz > 0) || !(dv > 0)) {
+    fail(SGDCT_EINVAL, "invalid cone-beam geometry arguments");
+    return nullptr;
+  }
+  if (nz > kMaxNz3D || nv > kMaxNv3D) {
+    fail(SGDCT_EUNSUP, "cone-beam projector supports nz <= 512 and nv <= 512");
+    return nullptr;
+  }
+  sgdct_projector* h = create(BASIS_SF, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, nullptr, 0.0, 0.5);
+  if (!h) return nullptr;
+  h->basis = BASIS_SF_3D;
+  h->cone_model = trap_v ? 2 : 1;
+  h->nz = nz;
+  h->nv = nv;
+  h->dz = dz;
+  h->dv = dv;
+  return h;
+}
+
+int sf3d_forward_projection(double* b, double* x, sgdct_projector* h) {
+  return (int)cone_host_forward(b, x, h, BASIS_SF_3D);
+}
+
+int sf3d_backward_projection(double* b, double* x, sgdct_projector* h) {
+  return cone_host_backward(b, x, h, BASIS_SF_3D);
+}
 
 int sgdct_shape3(const sgdct_projector* h, int* out) {
   if (!h || !out) return fail(SGDCT_EINVAL, "null argument");

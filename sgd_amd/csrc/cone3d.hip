@@ -1,4 +1,5 @@
-// b1_3d.hip -- gfx950 kernels of the cone-beam B1 projector (layers/A_B1_3D.cu).
+// cone3d.hip -- gfx950 kernels of the cone-beam projectors: B1 (layers/A_B1_3D.cu)
+// and separable footprints SF-TR / SF-TT (layers/SF_projector.cu).
 //
 // The reference's value of voxel (ix, iy, iz) in detector cell (u, j) of view p
 // factors into an in-plane part and an axial part (A_B1_3D.cu:95-305):
@@ -21,6 +22,19 @@
 //   axial_bwd_kernel   x[iz][c] += cfac * sum_j gamma_j Yb[c][j]
 // Volumes are kept column-major (xT[c][iz]) inside, transposed on the way in
 // and out, so a wave's voxel reads are contiguous.
+//
+// The separable-footprint projector (layers/SF_projector.cu:81-234) has the
+// same structure: its u trapezoid (corners ix +- 1/2, iy +- 1/2) does not
+// depend on iz, and its amplitude factors into an in-plane part
+// (2 / trapezoid width) and an axial part (lsd^2 dx dy dz / (du dv r^2) times
+// the inverse v-footprint area).  The kernels are templated on the model:
+//   CONE_B1     B1 in-plane, axial trapezoid from x +- 1, rows [j, j+1),
+//               gamma <= 1e-7 skipped (A_B1_3D.cu:300)
+//   CONE_SF_TR  SF trapezoid in-plane, axial rectangle of the centre column,
+//               cells centred on integers ([j - 1/2, j + 1/2], :209-216)
+//   CONE_SF_TT  as SF_TR with an axial trapezoid from x +- 1/2 at iy
+// For SF the view table is built with edges at half-integer detector
+// coordinates, so "bin u" of the in-plane machinery is cell [u - 1/2, u + 1/2].
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -28,6 +42,7 @@
 
 #include "footprint.hpp"
 #include "projector.hpp"
+#include "sf2d_dev.hpp"
 
 namespace sgdct {
 
@@ -35,21 +50,28 @@ constexpr int kMaxNz = 512;        // voxels per column staged in LDS
 constexpr int kMaxRowChunks = 8;   // detector rows per lane: nv <= 512
 constexpr float kGammaEps = 1e-7f; // A_B1_3D.cu:300 skips gamma <= eps
 
-// Axial trapezoid of voxel iz in a column: rows of the four corners
-// (x index +-1 at the column's iy, z index +-1/2; A_B1_3D.cu:133-137,
-// 188-202), sorted, and cfac (A_B1_3D.cu:209-210).
+enum ConeModel : int { CONE_B1 = 0, CONE_SF_TR = 1, CONE_SF_TT = 2 };
+
+// Axial footprint of voxel iz in a column: sorted rows of its corners (B1:
+// x index +- 1 at the column's iy, A_B1_3D.cu:133-137, 188-202; SF-TT:
+// x +- 1/2, SF_projector.cu:136-141, 160-168; SF-TR: the centre column, a
+// rectangle stored as v0 = v1, v2 = v3, :151-158) and the axial factor cf
+// (B1: dx dy dz * path-length weight * 2 / widths, A_B1_3D.cu:209-210;
+// SF: lsd^2 dx dy dz / (du dv r^2) / area, SF_projector.cu:193-198, 364).
 struct AxialTrap {
   float v0, v1, v2, v3, cf;
 };
 
 struct ColumnAxial {
-  float ka, kb;   // ldz / depth at x index ix - 1 and ix + 1
-  float inv_r2;   // 1 / |in-plane source -> centre|^2
+  float ka, kb;   // ldz / depth at the two x corners (SF-TR: both the centre)
+  float r2;       // |in-plane source -> centre|^2
 };
 
+template <int M>
 __device__ __forceinline__ ColumnAxial column_axial(const Cone3DArgs& a, const ViewGeo& vg, int ix, int iy) {
+  const double off = (M == CONE_B1) ? 1.0 : (M == CONE_SF_TT ? 0.5 : 0.0);
   const double wy = iy * a.g.dy + vg.w0y;
-  const double wxa = (ix - 1) * a.g.dx + vg.w0x, wxb = (ix + 1) * a.g.dx + vg.w0x;
+  const double wxa = (ix - off) * a.g.dx + vg.w0x, wxb = (ix + off) * a.g.dx + vg.w0x;
   const double wx = ix * a.g.dx + vg.w0x;
   // depth along the central axis = -cross(pu, X - p) / |pu|
   const double da = -(vg.pux * wy - vg.puy * wxa) / a.du;
@@ -57,10 +79,11 @@ __device__ __forceinline__ ColumnAxial column_axial(const Cone3DArgs& a, const V
   ColumnAxial c;
   c.ka = (float)(a.ldz / da);
   c.kb = (float)(a.ldz / db);
-  c.inv_r2 = (float)(1.0 / (wx * wx + wy * wy));
+  c.r2 = (float)(wx * wx + wy * wy);
   return c;
 }
 
+template <int M>
 __device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const ColumnAxial& c, int iz) {
   const float zc = (float)iz - 0.5f * (float)(a.nz - 1);
   const float a1 = __builtin_fmaf(c.ka, zc - 0.5f, a.vc), a2 = __builtin_fmaf(c.ka, zc + 0.5f, a.vc);
@@ -72,9 +95,24 @@ __device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const Colum
   t.v1 = fminf(h1, l2);
   t.v2 = fmaxf(h1, l2);
   const float oz = zc * a.dz;
-  const float w = __builtin_sqrtf(__builtin_fmaf(oz * oz, c.inv_r2, 1.0f));
-  t.cf = a.cvol * w * 2.0f / ((t.v3 - t.v0) + (t.v2 - t.v1));
+  if (M == CONE_B1) {
+    const float w = __builtin_sqrtf(__builtin_fmaf(oz * oz, 1.0f / c.r2, 1.0f));
+    t.cf = a.cvol * w * 2.0f / ((t.v3 - t.v0) + (t.v2 - t.v1));
+  } else {
+    if (M == CONE_SF_TR) {  // a1 == b1, a2 == b2: a rectangle
+      t.v1 = t.v0;
+      t.v2 = t.v3;
+    }
+    const float area = (M == CONE_SF_TR) ? (t.v3 - t.v0) : 0.5f * ((t.v3 - t.v0) + (t.v2 - t.v1));
+    t.cf = a.fac / ((c.r2 + oz * oz) * area);
+  }
   return t;
+}
+
+// detector row j spans [j + row_off, j + row_off + 1)
+template <int M>
+__device__ __forceinline__ float row_off() {
+  return M == CONE_B1 ? 0.0f : -0.5f;
 }
 
 // Overlap of [s, s+1) with the unit-height trapezoid (A_SF.cu:48-84 /
@@ -92,10 +130,16 @@ __device__ __forceinline__ float trap_gamma(float s, const AxialTrap& t) {
   return g0 + g1 + g2;
 }
 
+template <int M>
+__device__ __forceinline__ bool keep_gamma(float gm) {
+  return M == CONE_B1 ? gm > kGammaEps : gm == gm;
+}
+
 // ---------------------------------------------------------------------------
 // Y[pl][c][j] = sum_iz xT[c][iz] * cfac * gamma_j.  Block: 4 waves, one
 // column each; grid (ceil(ncol / 4), views in the chunk).
 // ---------------------------------------------------------------------------
+template <int M>
 __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, const float* __restrict__ xT,
                                                         float* __restrict__ Y) {
   __shared__ float lds[4][5][kMaxNz];
@@ -105,9 +149,9 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
   const bool live = c < ncol;
   if (live) {
     const int ix = c % a.g.nx, iy = c / a.g.nx;
-    const ColumnAxial ca = column_axial(a, a.g.views[p0 + pl], ix, iy);
+    const ColumnAxial ca = column_axial<M>(a, a.g.views[p0 + pl], ix, iy);
     for (int iz = lane; iz < a.nz; iz += 64) {
-      const AxialTrap t = voxel_trap(a, ca, iz);
+      const AxialTrap t = voxel_trap<M>(a, ca, iz);
       lds[wv][0][iz] = t.v0;
       lds[wv][1][iz] = t.v1;
       lds[wv][2][iz] = t.v2;
@@ -119,8 +163,8 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
   if (!live) return;
   float* yrow = Y + ((size_t)pl * ncol + c) * a.nv;
   for (int j = lane; j < a.nv; j += 64) {
-    const float fj = (float)j;
-    // first voxel whose trapezoid reaches row j (v3 grows with iz)
+    const float fj = (float)j + row_off<M>();
+    // first voxel whose footprint reaches row j (v3 grows with iz)
     int lo = 0, hi = a.nz;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
@@ -135,7 +179,7 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
       t.v2 = lds[wv][2][iz];
       t.v3 = lds[wv][3][iz];
       const float gm = trap_gamma(fj, t);
-      if (gm > kGammaEps) acc = __builtin_fmaf(lds[wv][4][iz], gm, acc);
+      if (keep_gamma<M>(gm)) acc = __builtin_fmaf(lds[wv][4][iz], gm, acc);
     }
     yrow[j] = acc;
   }
@@ -148,6 +192,28 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
 // (consecutive blocks, which the dispatcher spreads over the XCDs, take
 // different views; the blocks of one view share an XCD's L2 when nview is 8).
 // ---------------------------------------------------------------------------
+// SF in-plane weight of cell u for pixel (ix, iy): area of the pixel's unit
+// trapezoid over the cell times 2 / (sum of its widths), from the corner
+// projections relative to edge u, -N_u(c) / beta(c) (SF_projector.cu:123-133,
+// 175-190, 193-198, 209-214).
+__device__ __forceinline__ float sf_cell_weight(const ViewGeo& vg, const Proj2DArgs& g, int u, int ix, int iy) {
+  const double wx = ix * g.dx + vg.w0x, wy = iy * g.dy + vg.w0y;
+  const double ex = vg.e0x + u * vg.pux, ey = vg.e0y + u * vg.puy;
+  const float nc = (float)(ex * wy - ey * wx);
+  const float bc = (float)(vg.pux * wy - vg.puy * wx);
+  const float hx = (float)(-0.5 * ey * g.dx), hy = (float)(0.5 * ex * g.dy);      // N_u steps
+  const float bx = (float)(-0.5 * vg.puy * g.dx), by = (float)(0.5 * vg.pux * g.dy);  // beta steps
+  const float q0 = -(nc - hx - hy) / (bc - bx - by);
+  const float q1 = -(nc - hx + hy) / (bc - bx + by);
+  const float q2 = -(nc + hx - hy) / (bc + bx - by);
+  const float q3 = -(nc + hx + hy) / (bc + bx + by);
+  const Trap t = make_trap(q0, q1, q2, q3);
+  const float width = (t.q3 - t.q0) + (t.q2 - t.q1);
+  const float gm = trap_gamma(t, 0.0f, 1.0f);
+  return width > 0.0f ? gm * 2.0f / width : 0.0f;
+}
+
+template <int M>
 __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int nview, const float* __restrict__ Y,
                                                        float* __restrict__ proj) {
   const Proj2DArgs& g = a.g;
@@ -158,7 +224,8 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
   const int v = p0 + pl;
   const int ncol = g.nx * g.ny;
   const ViewGeo vg = g.views[v];
-  const B1Bin rec = reinterpret_cast<const B1Bin*>(g.bins)[(size_t)v * g.nu + u];
+  B1Bin rec{};
+  if (M == CONE_B1) rec = reinterpret_cast<const B1Bin*>(g.bins)[(size_t)v * g.nu + u];
 
   const double eax = vg.e0x + u * vg.pux, eay = vg.e0y + u * vg.puy;
   const double ebx = eax + vg.pux, eby = eay + vg.puy;
@@ -166,9 +233,12 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
   const bool rows = fabs(my) >= fabs(mx);
   const double acx = -eay * g.dx, acy = eax * g.dy, ac0 = eax * vg.w0y - eay * vg.w0x;
   const double bcx = -eby * g.dx, bcy = ebx * g.dy, bc0 = ebx * vg.w0y - eby * vg.w0x;
-  // clamped end bins (A_B1_3D.cu:150-151, as in the 2-D kernel)
-  const double lax = (u == 0) ? eax - vg.pux : eax, lay = (u == 0) ? eay - vg.puy : eay;
-  const double lbx = (u == g.nu - 1) ? ebx + vg.pux : ebx, lby = (u == g.nu - 1) ? eby + vg.puy : eby;
+  // clamped end bins (A_B1_3D.cu:150-151, as in the 2-D kernel); SF drops
+  // cells outside the detector instead
+  const bool clamp = M == CONE_B1;
+  const double lax = (clamp && u == 0) ? eax - vg.pux : eax, lay = (clamp && u == 0) ? eay - vg.puy : eay;
+  const double lbx = (clamp && u == g.nu - 1) ? ebx + vg.pux : ebx;
+  const double lby = (clamp && u == g.nu - 1) ? eby + vg.puy : eby;
   const double qcx = -lay * g.dx, qcy = lax * g.dy, qc0 = lax * vg.w0y - lay * vg.w0x;
   const double rcx = -lby * g.dx, rcy = lbx * g.dy, rc0 = lbx * vg.w0y - lby * vg.w0x;
   const double ra = g.h * (fabs(qcx) + fabs(qcy)), rb = g.h * (fabs(rcx) + fabs(rcy));
@@ -212,8 +282,14 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
 #pragma unroll
     for (int k = 0; k < kMaxRowChunks; ++k) accl[k] = 0.0f;
     for (int n = n0; n <= n1; ++n) {
-      const float w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
-                                s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+      float w;
+      if (M == CONE_B1) {
+        w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
+                      s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+      } else {
+        const int ix = rows ? n : m, iy = rows ? m : n;
+        w = sf_cell_weight(vg, g, u, ix, iy);
+      }
       const float* yc = ybase + (size_t)(m * smaj + n * smin) * a.nv;
 #pragma unroll
       for (int k = 0; k < kMaxRowChunks; ++k)
@@ -236,6 +312,7 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
 // Yb[pl][c][j] = sum_u conv(u, c) projT[pl][u][j].  Wave per (column, view),
 // the column's clamped corner range (A_B1_3D.cu:117-151) wave-uniform.
 // ---------------------------------------------------------------------------
+template <int M>
 __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, const float* __restrict__ projT,
                                                        float* __restrict__ Yb) {
   const Proj2DArgs& g = a.g;
@@ -246,7 +323,7 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
   const int v = p0 + pl;
   const int ix = c % g.nx, iy = c / g.nx;
   const ViewGeo& vg = g.views[v];
-  const B1Bin* bins = reinterpret_cast<const B1Bin*>(g.bins) + (size_t)v * g.nu;
+  const B1Bin* bins = M == CONE_B1 ? reinterpret_cast<const B1Bin*>(g.bins) + (size_t)v * g.nu : nullptr;
   const double wx = ix * g.dx + vg.w0x, wy = iy * g.dy + vg.w0y;
   const double n0 = vg.e0x * wy - vg.e0y * wx;
   const double be = vg.pux * wy - vg.puy * wx;
@@ -262,11 +339,16 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
   for (int k = 0; k < kMaxRowChunks; ++k) acc[k] = 0.0f;
   float na = (float)fma((double)ulo, be, n0);
   for (int u = ulo; u <= uhi; ++u) {
-    const B1Bin rec = bins[u];
     const float nb = na + bef;
-    const float ta = rec.ka * na, tb = rec.kb * nb;
-    const float w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
-                              s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+    float w;
+    if (M == CONE_B1) {
+      const B1Bin rec = bins[u];
+      const float ta = rec.ka * na, tb = rec.kb * nb;
+      w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
+                    s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+    } else {
+      w = sf_cell_weight(vg, g, u, ix, iy);
+    }
     const float* row = projT + ((size_t)pl * g.nu + u) * a.nv + lane;
 #pragma unroll
     for (int k = 0; k < kMaxRowChunks; ++k)
@@ -283,6 +365,7 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
 // xT[c][iz] (+)= sum over the chunk's views of cfac * sum_j gamma_j Yb[c][j].
 // Wave per column, lanes over voxels; grid ceil(ncol / 4).
 // ---------------------------------------------------------------------------
+template <int M>
 __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, int nview, const float* __restrict__ Yb,
                                                         float* __restrict__ xT, int accumulate) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -296,19 +379,20 @@ __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, in
   for (int s = 0; s < kSlots; ++s) acc[s] = 0.0;
   const float lastj = (float)(a.nv - 1);
   for (int pl = 0; pl < nview; ++pl) {
-    const ColumnAxial ca = column_axial(a, a.g.views[p0 + pl], ix, iy);
+    const ColumnAxial ca = column_axial<M>(a, a.g.views[p0 + pl], ix, iy);
     const float* yrow = Yb + ((size_t)pl * ncol + c) * a.nv;
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
       const int iz = s * 64 + lane;
       if (iz >= a.nz) continue;
-      const AxialTrap t = voxel_trap(a, ca, iz);
-      const int jlo = (int)fminf(fmaxf(floorf(t.v0), 0.0f), lastj);
-      const int jhi = (int)fminf(fmaxf(floorf(t.v3), 0.0f), lastj);
+      const AxialTrap t = voxel_trap<M>(a, ca, iz);
+      const float ro = row_off<M>();
+      const int jlo = (int)fminf(fmaxf(floorf(t.v0 - ro), 0.0f), lastj);
+      const int jhi = (int)fminf(fmaxf(floorf(t.v3 - ro), 0.0f), lastj);
       float sum = 0.0f;
       for (int j = jlo; j <= jhi; ++j) {
-        const float gm = trap_gamma((float)j, t);
-        if (gm > kGammaEps) sum = __builtin_fmaf(gm, yrow[j], sum);
+        const float gm = trap_gamma((float)j + ro, t);
+        if (keep_gamma<M>(gm)) sum = __builtin_fmaf(gm, yrow[j], sum);
       }
       acc[s] += (double)(t.cf * sum);
     }
@@ -347,19 +431,37 @@ hipError_t launch_transpose(const float* in, float* out, int rows, int cols, int
   return hipGetLastError();
 }
 
-hipError_t launch_b1_3d_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y,
-                                      float* proj, hipStream_t s) {
+template <int M>
+static void cone_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y, float* proj,
+                               hipStream_t s) {
   const int ncol = a.g.nx * a.g.ny;
-  axial_fwd_kernel<<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, xT, Y);
-  conv_fwd_kernel<<<dim3(((a.g.nu + 3) / 4) * nview), dim3(256), 0, s>>>(a, p0, nview, Y, proj);
+  axial_fwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, xT, Y);
+  conv_fwd_kernel<M><<<dim3(((a.g.nu + 3) / 4) * nview), dim3(256), 0, s>>>(a, p0, nview, Y, proj);
+}
+
+template <int M>
+static void cone_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb, float* xT,
+                                int accumulate, hipStream_t s) {
+  const int ncol = a.g.nx * a.g.ny;
+  conv_bwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, projT, Yb);
+  axial_bwd_kernel<M><<<dim3((ncol + 3) / 4), dim3(256), 0, s>>>(a, p0, nview, Yb, xT, accumulate);
+}
+
+hipError_t launch_cone_forward_chunk(int model, const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y,
+                                     float* proj, hipStream_t s) {
+  if (model == CONE_B1) cone_forward_chunk<CONE_B1>(a, p0, nview, xT, Y, proj, s);
+  else if (model == CONE_SF_TR) cone_forward_chunk<CONE_SF_TR>(a, p0, nview, xT, Y, proj, s);
+  else if (model == CONE_SF_TT) cone_forward_chunk<CONE_SF_TT>(a, p0, nview, xT, Y, proj, s);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_b1_3d_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb,
-                                       float* xT, int accumulate, hipStream_t s) {
-  const int ncol = a.g.nx * a.g.ny;
-  conv_bwd_kernel<<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, projT, Yb);
-  axial_bwd_kernel<<<dim3((ncol + 3) / 4), dim3(256), 0, s>>>(a, p0, nview, Yb, xT, accumulate);
+hipError_t launch_cone_backward_chunk(int model, const Cone3DArgs& a, int p0, int nview, const float* projT,
+                                      float* Yb, float* xT, int accumulate, hipStream_t s) {
+  if (model == CONE_B1) cone_backward_chunk<CONE_B1>(a, p0, nview, projT, Yb, xT, accumulate, s);
+  else if (model == CONE_SF_TR) cone_backward_chunk<CONE_SF_TR>(a, p0, nview, projT, Yb, xT, accumulate, s);
+  else if (model == CONE_SF_TT) cone_backward_chunk<CONE_SF_TT>(a, p0, nview, projT, Yb, xT, accumulate, s);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

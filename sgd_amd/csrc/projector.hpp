@@ -20,7 +20,9 @@
 
 namespace sgdct {
 
-enum Basis : int { BASIS_B0 = 0, BASIS_B1 = 1, BASIS_SF = 2, BASIS_H0 = 3, BASIS_HL = 4, BASIS_B1_3D = 5 };
+enum Basis : int {
+  BASIS_B0 = 0, BASIS_B1 = 1, BASIS_SF = 2, BASIS_H0 = 3, BASIS_HL = 4, BASIS_B1_3D = 5, BASIS_SF_3D = 6
+};
 
 // Per-view FP64 constants.  With w = X - p = (ix dx + w0x, iy dy + w0y):
 //   N_k(ix, iy) = cross(e0 + k pu, w),   beta(ix, iy) = cross(pu, w) (< 0).
@@ -53,6 +55,7 @@ struct Cone3DArgs {
   float cvol;       // dx * dy * dz
   float dz;
   double du;        // |pu|
+  float fac;        // separable footprints: lsd^2 dx dy dz / (du dv)
 };
 
 // One slice of a batched persistent launch (batch.hip): a 2-D projector's

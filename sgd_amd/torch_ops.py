@@ -65,13 +65,16 @@ class Projector:
         return cls(basis, nc, nl, angles, nu, dt, dt, du, lsd, lso, drho=drho)
 
     @classmethod
-    def cone_beam(cls, nx, ny, nz, angles, nu, nv, dx, dy, dz, du, dv, lsd, lso):
-        """Cone-beam B1 (b1_3d_init): volumes (nz, ny, nx), projections
-        (np, nv, nu); A^T is the transpose of A."""
+    def cone_beam(cls, nx, ny, nz, angles, nu, nv, dx, dy, dz, du, dv, lsd, lso, model="B1"):
+        """Cone-beam projector, model "B1" (b1_3d_init), "SF-TR" or "SF-TT"
+        (sf3d_init): volumes (nz, ny, nx), projections (np, nv, nu); A^T is
+        the transpose of A."""
+        if model not in ("B1", "SF-TR", "SF-TT"):
+            raise ValueError("cone-beam models are B1, SF-TR and SF-TT")
         if not torch.cuda.is_available():
             raise _lib.SgdctError("sgd_amd projectors need a ROCm GPU (torch.cuda unavailable)")
         self = cls.__new__(cls)
-        self.basis = "B1_3D"
+        self.basis = "B1_3D" if model == "B1" else model
         self.nx, self.ny, self.nz, self.nu, self.nv = int(nx), int(ny), int(nz), int(nu), int(nv)
         ang = np.ascontiguousarray(angles, dtype=np.float64)
         self.np = int(ang.size)
@@ -79,10 +82,15 @@ class Projector:
         self.geom = dict(dx=float(dx), dy=float(dy), dz=float(dz), du=float(du), dv=float(dv), lsd=float(lsd),
                          lso=float(lso))
         torch.cuda.init()
-        h = _lib.lib().b1_3d_init(self.nx, self.ny, self.nz, self.np, self.nu, self.nv, dx, dy, dz, du, dv, lsd, lso,
-                                  _lib.dptr(ang))
+        L = _lib.lib()
+        if model == "B1":
+            h = L.b1_3d_init(self.nx, self.ny, self.nz, self.np, self.nu, self.nv, dx, dy, dz, du, dv, lsd, lso,
+                             _lib.dptr(ang))
+        else:
+            h = L.sf3d_init(self.nx, self.ny, self.nz, self.np, self.nu, self.nv, dx, dy, dz, du, dv, lsd, lso,
+                            _lib.dptr(ang), int(model == "SF-TT"))
         if not h:
-            raise _lib.SgdctError("b1_3d_init failed: %s" % _lib.last_error())
+            raise _lib.SgdctError("cone-beam init failed: %s" % _lib.last_error())
         self.handle = h
         self.device = torch.device("cuda", torch.cuda.current_device())
         self._img_shape = (self.nz, self.ny, self.nx)
