@@ -1,8 +1,9 @@
 """The warm-start U-Net's gfx950 kernels (sgd_amd/csrc/unet.hip) against the
 CPU oracle (oracle/unet_oracle.py, pinned by finite differences):
   * forward output and every parameter gradient equal the oracle's, which
-    rounds to bfloat16 at the same points (relative L2 <= 1e-2 / 2e-2), and
-    stay within bfloat16 precision of the oracle's unrounded FP64 network;
+    rounds to bfloat16 at the same points (relative L2 <= 1e-2 for the output,
+    <= 4e-2 per layer and <= 1.5e-2 over all gradients), and stay within
+    bfloat16 precision of the oracle's unrounded FP64 network;
   * a few Adam steps on a fixed batch lower the loss.
 """
 import numpy as np
@@ -32,7 +33,10 @@ def test_forward_and_gradients_match_oracle(cuda, levels, c0, hw):
     torch.cuda.synchronize()
     yg = y.cpu().numpy()
     gg = net.grads_numpy()
-    for use_bf16, tol_y, tol_g in ((True, 1e-2, 2e-2), (False, 3e-2, 8e-2)):
+    # per-layer bars allow for bfloat16 rounding flips (one ulp, 2^-8, of an
+    # activation or gradient element rounded the other way), which weigh most
+    # in the deepest layers' few pixels; the whole gradient vector is tighter
+    for use_bf16, tol_y, tol_g, tol_all in ((True, 1e-2, 4e-2, 1.5e-2), (False, 3e-2, 8e-2, 4e-2)):
         o = UNetOracle(params, levels, c0, use_bf16=use_bf16)
         yo, tape = o.forward(x)
         go = o.backward(tape, yo, target)
@@ -45,7 +49,10 @@ def test_forward_and_gradients_match_oracle(cuda, levels, c0, hw):
                 e = _rel(gg[name][k], go[name][k])
                 worst = max(worst, e)
                 assert e <= tol_g, (name, k, e)
-        print("  worst gradient rel %.2e" % worst)
+        flat = lambda g: np.concatenate([np.ravel(g[n][k]) for n in sorted(g) for k in (0, 1)])
+        e_all = _rel(flat(gg), flat(go))
+        print("  worst layer gradient rel %.2e, all gradients rel %.2e" % (worst, e_all))
+        assert e_all <= tol_all
 
 
 def test_adam_lowers_the_loss(cuda):
