@@ -126,11 +126,24 @@ void sgdct_destroy(sgdct_projector *h);
 /* Image (ny, nx) FP32 row-major [iy][ix] and sinogram (np, nu) FP32 [p][u]
  * (cone beam: volume (nz, ny, nx) and projections (np, nv, nu)), both device
  * pointers; `batch` images / sinograms back to back; `stream` is
- * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.  A cone-beam
- * handle keeps per-handle workspace: its calls must not overlap.
+ * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.  Cone-beam
+ * handles, and B0 / B1 handles on launches they split (below), keep per-handle
+ * workspace: calls on one handle must not overlap (issue them on one stream).
  * Returns SGDCT_OK or a negative code. */
 int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
+
+/* B0 / B1 launch shape (no reference counterpart).  A splits every bin's
+ * pixel walk into `fwd_split` line segments and A^T the views into
+ * `bwd_split` groups, each writing FP64 partial sums that a second kernel adds
+ * in a fixed order (results are deterministic, independent of the batch size,
+ * and agree across splits to FP64 rounding).  0 = automatic: chosen at
+ * creation from one image's launch size (a launch of few waves splits, up to
+ * 16 ways); 1..16 = fixed.  The partial-sum workspace (up to 4 images per
+ * launch; larger batches take several launches) is allocated at creation and
+ * by this call, never inside a projection.  Returns SGDCT_OK, SGDCT_EINVAL or
+ * SGDCT_ENOMEM. */
+int sgdct_split_config(sgdct_projector *h, int fwd_split, int bwd_split);
 
 /* Shape query: out[0..3] = nx, ny, np, nu (hexagonal: nc, nl, np, nu);
  * returns the basis id (0 B0, 1 B1, 2 SF, 3 H0, 4 Hl) or a negative code. */

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void batch_fwd_kernel(BatchPlan P, const float
   }
 }
 
-__global__ __launch_bounds__(256) void batch_bwd_kernel(BatchPlan P, const float* __restrict__ sino,
+__global__ __launch_bounds__(256, 6) void batch_bwd_kernel(BatchPlan P, const float* __restrict__ sino,
                                                         float* __restrict__ img) {
   __shared__ B1Bin recs[kViewChunk][kMaxWindow];  // also holds B0Bin records
   __shared__ int win[2 * kViewChunk];

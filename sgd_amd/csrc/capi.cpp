@@ -16,10 +16,10 @@
 #include "projector.hpp"
 
 namespace sgdct {
-hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, float* sino, int batch,
-                          hipStream_t s);
+hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, float* sino, int batch, int nseg,
+                          double* part, hipStream_t s);
 hipError_t launch_backward(int basis, const Proj2DArgs& g, bool use_lds, const float* sino, float* img, int batch,
-                           hipStream_t s);
+                           int ngroup, double* part, hipStream_t s);
 int max_tile_window(const Proj2DArgs& g, const ViewGeo* views_host);
 hipError_t launch_count(const Proj2DArgs& g, unsigned long long* out, hipStream_t s);
 hipError_t launch_sf_forward(const Proj2DArgs& g, const float* img, float* sino, int batch, hipStream_t s);
@@ -125,6 +125,16 @@ struct sgdct_projector {
   unsigned long long* d_count = nullptr;
   long long evals = -1;
   bool bwd_lds = false;  // A^T stages bin windows in LDS (B0 / B1)
+  // B0 / B1: A splits each bin's walk into line segments and A^T the views
+  // into groups; fwd_auto / bwd_auto are chosen at creation from one image's
+  // launch size, fwd_split / bwd_split (0 = automatic) override them.  The
+  // FP64 partial sums go to d_part (part_cap doubles, allocated at creation
+  // or by sgdct_split_config, never inside a projection call); a batch is
+  // processed in as many launches as the workspace requires, so every image
+  // is computed the same way whatever the batch size.
+  int fwd_split = 0, bwd_split = 0, fwd_auto = 1, bwd_auto = 1;
+  double* d_part = nullptr;
+  size_t part_cap = 0;
   Proj2DArgs args{};
   // cone beam (BASIS_B1_3D): volume nz slices, detector nv rows
   int nz = 1, nv = 1;
@@ -141,6 +151,7 @@ struct sgdct_projector {
     if (d_img) (void)hipFree(d_img);
     if (d_sino) (void)hipFree(d_sino);
     if (d_count) (void)hipFree(d_count);
+    if (d_part) (void)hipFree(d_part);
     if (d_xT) (void)hipFree(d_xT);
     if (d_Y) (void)hipFree(d_Y);
     if (d_projT) (void)hipFree(d_projT);
@@ -184,6 +195,13 @@ struct sgdct_batch {
 
 namespace {
 
+int auto_fwd_split(const sgdct_projector* h);
+int auto_bwd_split(const sgdct_projector* h);
+int ensure_part(sgdct_projector* h, size_t doubles);
+size_t part_need(const sgdct_projector* h);
+// images per launch the partial-sum workspace holds
+constexpr int kPartImages = 4;
+
 // Per-(view, bin) constants, FP64 on the host, stored FP32 like the
 // reference's per-bin vectors (layers/A_B1.cu:429-478, A_B0.cu:338-388).
 void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, void* out) {
@@ -196,27 +214,28 @@ void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, vo
     const double th = std::atan2(std::fabs(cx), std::fabs(cy));
     const float vx = (float)std::fabs(dx * std::cos(th)), vy = (float)std::fabs(dy * std::sin(th));
     if (basis == BASIS_B1) {
+      // constants in the scaled offset t' = s t (footprint.hpp)
       B1Bin r{};
-      r.ka = (float)ka;
-      r.kb = (float)kb;
       const double a = std::max(vx, vy);
       double b = std::min(vx, vy);
+      double sc;
       if (b > 1e-7) {
-        const double c = 1.0 / (24.0 * a * a * b * b);
-        r.k2 = (float)(1.0 / (2.0 * a * a));
-        r.k4 = (float)(b * b / (12.0 * a * a));
-        r.cq = (float)c;
-        r.c2 = (float)(2.0 * c);
+        sc = std::pow(24.0 * a * a * b * b, -0.25);  // c^(1/4)
+        const double bs = b * sc;
+        r.k2 = (float)(12.0 * bs * bs);
+        r.k4 = (float)(2.0 * bs * bs * bs * bs);
       } else {  // 1tri1box limit, spline_conv.hh:393-405
         b = 0.0;
-        r.k2 = (float)(1.0 / (2.0 * a * a));
+        sc = 1.0 / a;
+        r.k2 = 0.5f;
         r.k4 = 0.0f;
-        r.cq = 0.0f;
-        r.c2 = 0.0f;
       }
-      r.a = (float)a;
-      r.b = (float)b;
-      r.apb = (float)(a + b);
+      r.ka = (float)(ka * sc);
+      r.kb = (float)(kb * sc);
+      r.a = (float)(a * sc);
+      r.b = (float)(b * sc);
+      r.apb = (float)((a + b) * sc);
+      r.scale = (float)sc;
       static_cast<B1Bin*>(out)[u] = r;
     } else {
       B0Bin r{};
@@ -306,7 +325,15 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   a.h = h->h;
   a.nchunk = (nu + 63) / 64;
   // 2 bins of slack for FP64 contraction differences between host and device
-  if (basis == BASIS_B0 || basis == BASIS_B1) h->bwd_lds = max_tile_window(a, views.data()) <= kMaxWindow - 2;
+  if (basis == BASIS_B0 || basis == BASIS_B1) {
+    h->bwd_lds = max_tile_window(a, views.data()) <= kMaxWindow - 2;
+    h->fwd_auto = auto_fwd_split(h);
+    h->bwd_auto = auto_bwd_split(h);
+    if (ensure_part(h, kPartImages * part_need(h)) != SGDCT_OK) {
+      delete h;
+      return nullptr;
+    }
+  }
   return h;
 }
 
@@ -393,21 +420,81 @@ int cone_backward(sgdct_projector* h, const float* proj, float* vol, int batch, 
   return SGDCT_OK;
 }
 
+// Launch-size targets of the automatic splits (one image): enough waves for
+// every SIMD to hold several (A's per-wave work is uneven, A^T's is even).
+constexpr long long kFwdWaveTarget = 65536, kBwdWaveTarget = 32768;
+constexpr int kMaxSplit = 16;
+
+int auto_fwd_split(const sgdct_projector* h) {
+  const long long waves = (long long)h->np * h->args.nchunk;
+  return (int)std::max(1LL, std::min<long long>(kMaxSplit, (kFwdWaveTarget + waves - 1) / waves));
+}
+
+int auto_bwd_split(const sgdct_projector* h) {
+  const long long waves = 4LL * ((h->nx + kTile - 1) / kTile) * ((h->ny + kTile - 1) / kTile);
+  const long long by_views = std::max(1, h->np / (4 * kViewChunk));  // groups of >= 16 views
+  return (int)std::max(1LL, std::min<long long>(std::min<long long>(kMaxSplit, by_views),
+                                                (kBwdWaveTarget + waves - 1) / waves));
+}
+
+// Room for `doubles` partial sums.
+int ensure_part(sgdct_projector* h, size_t doubles) {
+  if (doubles <= h->part_cap) return SGDCT_OK;
+  if (h->d_part) (void)hipFree(h->d_part);
+  h->d_part = nullptr;
+  h->part_cap = 0;
+  hipError_t e = hipMalloc(&h->d_part, doubles * sizeof(double));
+  if (e != hipSuccess) return hip_fail(e, "partial-sum workspace");
+  h->part_cap = doubles;
+  return SGDCT_OK;
+}
+
+// Workspace of one image at the current splits.
+size_t part_need(const sgdct_projector* h) {
+  const int fs = h->fwd_split ? h->fwd_split : h->fwd_auto;
+  const int bs = h->bwd_split ? h->bwd_split : h->bwd_auto;
+  return std::max(fs > 1 ? (size_t)fs * h->np * h->nu : 0, bs > 1 ? (size_t)bs * h->nx * h->ny : 0);
+}
+
 int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
   if (is_cone(h->basis)) return cone_forward(h, img, sino, batch, s);
-  hipError_t e;
+  hipError_t e = hipSuccess;
   if (is_hex(h->basis)) e = launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, s);
   else if (h->basis == BASIS_SF) e = launch_sf_forward(h->args, img, sino, batch, s);
-  else e = launch_forward(h->basis, h->args, img, sino, batch, s);
+  else {
+    const int nseg = h->fwd_split ? h->fwd_split : h->fwd_auto;
+    const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)nseg * ns;
+    if (nseg == 1) {
+      e = launch_forward(h->basis, h->args, img, sino, batch, 1, nullptr, s);
+    } else {
+      if (h->part_cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
+      const int bc = (int)std::min<size_t>(batch, h->part_cap / per);
+      for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
+        e = launch_forward(h->basis, h->args, img + b0 * ni, sino + b0 * ns, std::min(bc, batch - b0), nseg,
+                           h->d_part, s);
+    }
+  }
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "forward launch");
 }
 
 int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
   if (is_cone(h->basis)) return cone_backward(h, sino, img, batch, s);
-  hipError_t e;
+  hipError_t e = hipSuccess;
   if (is_hex(h->basis)) e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
   else if (h->basis == BASIS_SF) e = launch_sf_backward(h->args, sino, img, batch, s);
-  else e = launch_backward(h->basis, h->args, h->bwd_lds, sino, img, batch, s);
+  else {
+    const int ng = h->bwd_split ? h->bwd_split : h->bwd_auto;
+    const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)ng * ni;
+    if (ng == 1) {
+      e = launch_backward(h->basis, h->args, h->bwd_lds, sino, img, batch, 1, nullptr, s);
+    } else {
+      if (h->part_cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
+      const int bc = (int)std::min<size_t>(batch, h->part_cap / per);
+      for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
+        e = launch_backward(h->basis, h->args, h->bwd_lds, sino + b0 * ns, img + b0 * ni, std::min(bc, batch - b0),
+                            ng, h->d_part, s);
+    }
+  }
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "backward launch");
 }
 
@@ -928,6 +1015,21 @@ int sgdct_batch_graph_launch(sgdct_batch* b, int direction, const float* in, flo
   }
   hipError_t e = hipGraphLaunch(g.exec, s);
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "graph launch");
+}
+
+int sgdct_split_config(sgdct_projector* h, int fwd_split, int bwd_split) {
+  if (!h || fwd_split < 0 || fwd_split > kMaxSplit || bwd_split < 0 || bwd_split > kMaxSplit)
+    return fail(SGDCT_EINVAL, "split_config: splits must be 0 (automatic) or 1..16");
+  if (h->basis != BASIS_B0 && h->basis != BASIS_B1) return fail(SGDCT_EINVAL, "split_config: B0 / B1 handles only");
+  const int fs = h->fwd_split, bs = h->bwd_split;
+  h->fwd_split = fwd_split;
+  h->bwd_split = bwd_split;
+  if (int rc = ensure_part(h, kPartImages * part_need(h))) {
+    h->fwd_split = fs;
+    h->bwd_split = bs;
+    return rc;
+  }
+  return SGDCT_OK;
 }
 
 int sgdct_forward_dev(sgdct_projector* h, const float* img, float* sino, int batch, void* stream) {

@@ -284,8 +284,8 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
     for (int n = n0; n <= n1; ++n) {
       float w;
       if (M == CONE_B1) {
-        w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
-                      s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+        w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4),
+                      s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4), ta, tb) * rec.scale;
       } else {
         const int ix = rows ? n : m, iy = rows ? m : n;
         w = sf_cell_weight(vg, g, u, ix, iy);
@@ -344,8 +344,8 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
     if (M == CONE_B1) {
       const B1Bin rec = bins[u];
       const float ta = rec.ka * na, tb = rec.kb * nb;
-      w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2),
-                    s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4, rec.cq, rec.c2), ta, tb);
+      w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4),
+                    s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4), ta, tb) * rec.scale;
     } else {
       w = sf_cell_weight(vg, g, u, ix, iy);
     }

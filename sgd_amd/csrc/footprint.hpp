@@ -21,12 +21,23 @@ namespace sgdct {
 //        = 12 b^2 (y-b)^2 + 2 b^4           for y >= 2b (expanded: no cancellation)
 // For b <= 1e-7 the reference switches to tri_a * box (spline_conv.hh:393-405),
 // which is the b -> 0 limit of the same expression (c terms dropped).
+//
+// The kernels work in the scaled offset t' = s t with s = c^(1/4) (s = 1/a in
+// the tri_a * box limit), which takes c out of every evaluation: with
+// y = a' + b' - |t'|, a' = s a, b' = s b,
+//   S(t) = sign(t') * (1/2 - Q(y) + 2 max(y - a', 0)^4)
+//   Q(y) = y^4 - 2 max(y - b', 0)^4              for y < 2b'
+//        = k2 (y - b')^2 + k4                    for y >= 2b'
+// k2 = 12 b'^2, k4 = 2 b'^4 (limit: b' = 0, k2 = 1/2, k4 = 0; there y <= a',
+// so the last term of S vanishes), and
+//   conv = (S(tb) - S(ta)) / (tb - ta) = s (S(tb') - S(ta')) / (tb' - ta').
+// Traits<B1>::conv returns the quotient without the factor s; A applies
+// `scale` once per bin, A^T stages it into the sinogram value.
 struct B1Bin {
-  float ka, kb;      // ta = ka * N_u(pixel), tb = kb * N_{u+1}(pixel)
-  float apb, b, a;   // a + b, b, a
-  float k2, k4;      // c*12 b^2 = 1/(2a^2),  c*2 b^4 = b^2/(12 a^2)
-  float cq, c2;      // c, 2c (0 when b <= 1e-7)
-  float pad0, pad1, pad2;
+  float ka, kb;      // ta' = ka * N_u(pixel), tb' = kb * N_{u+1}(pixel)
+  float apb, b, a;   // a' + b', b', a'
+  float k2, k4;      // see above
+  float scale;       // s (A^T's LDS copy holds sinogram value * s here)
 };
 
 // Per-bin constants of the B0 kernel K = box_p * box_q (trapezoid, p >= q;
@@ -37,15 +48,15 @@ struct B0Bin {
   float ka, kb;
   float hpq, hmq;    // (p+q)/2, (p-q)/2
   float inv_p, inv_2pq;
-  float pad0, pad1;
+  float y;           // A^T's LDS copy: sinogram value
+  float pad;
 };
 
 __device__ __forceinline__ float copysign_f(float mag, float sgn) {
   return __builtin_copysignf(mag, sgn);
 }
 
-__device__ __forceinline__ float s_b1(float t, float apb, float b, float a, float k2, float k4,
-                                      float cq, float c2) {
+__device__ __forceinline__ float s_b1(float t, float apb, float b, float a, float k2, float k4) {
   float x = __builtin_fabsf(t);
   float y = __builtin_fmaxf(apb - x, 0.0f);
   float v = y - b;
@@ -53,11 +64,11 @@ __device__ __forceinline__ float s_b1(float t, float apb, float b, float a, floa
   float y2 = y * y;
   float m = __builtin_fmaxf(v, 0.0f);
   float m2 = m * m;
-  float qb = cq * __builtin_fmaf(-2.0f, m2 * m2, y2 * y2);
-  float cqv = (v >= b) ? qa : qb;
+  float qb = __builtin_fmaf(-2.0f, m2 * m2, y2 * y2);
+  float q = (v >= b) ? qa : qb;
   float e = __builtin_fmaxf(y - a, 0.0f);
   float e2 = e * e;
-  float s = __builtin_fmaf(c2, e2 * e2, 0.5f - cqv);
+  float s = __builtin_fmaf(2.0f, e2 * e2, 0.5f - q);
   return copysign_f(s, t);
 }
 

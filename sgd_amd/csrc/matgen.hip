@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void matgen_kernel(Proj2DArgs g, unsigned int*
   // edge-ray crossings sit at ta * n and tb * n from the pixel centre
   const double cx = eax + 0.5 * vg.pux, cy = eay + 0.5 * vg.puy, cl = sqrt(cx * cx + cy * cy);
   const float nxa = (float)(fabs(cy) / cl), nya = (float)(fabs(cx) / cl);
-  const float fdx = (float)g.dx, fdy = (float)g.dy;
+  // (ta, tb below are in the basis's scaled offset: compare with dx, dy scaled alike)
+  const float fdx = (float)g.dx * T::scale(rec), fdy = (float)g.dy * T::scale(rec);
   // pixel set: corner rule with half-extent h + 1 (the footprint's support
   // is the projection of the basis at the pixel's depth, within one pixel of
   // the perspective corner range); walk columns ix, rows iy inside
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(256) void matgen_kernel(Proj2DArgs g, unsigned int*
       const float ta = (float)(rec.ka * na), tb = (float)(rec.kb * nb);
       const float fa = __builtin_fabsf(ta), fb = __builtin_fabsf(tb);
       if (fa * nxa > fdx && fb * nxa > fdx && fa * nya > fdy && fb * nya > fdy) continue;
-      const float w = T::conv(rec, ta, tb);
+      const float w = T::conv(rec, ta, tb) * T::scale(rec);
       if (!(w > 0.0f)) continue;
       if (WRITE) {
         if (cnt >= cap) break;

@@ -14,6 +14,7 @@
 // (B0), layers/A_SF.cu:48-225 (SF); see footprint.hpp for the form used.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -21,11 +22,26 @@
 
 namespace sgdct {
 
+// grid (waves / 4, batch, nseg).  With nseg > 1 every (segment, image)
+// writes FP64 partial sums to part[(seg * batch + image) * np * nu + ...],
+// which sum_parts_kernel adds in segment order.
 template <int BASIS>
 __global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float* __restrict__ img,
-                                                      float* __restrict__ sino) {
-  fwd_bin_wave<BASIS>(g, img + (size_t)blockIdx.y * g.nx * g.ny, sino + (size_t)blockIdx.y * g.np * g.nu,
-                      blockIdx.x * 4 + (threadIdx.x >> 6));
+                                                      float* __restrict__ sino, double* __restrict__ part) {
+  const int b = blockIdx.y, seg = blockIdx.z, nseg = gridDim.z;
+  const size_t nsino = (size_t)g.np * g.nu;
+  fwd_bin_wave<BASIS>(g, img + (size_t)b * g.nx * g.ny, sino + b * nsino, blockIdx.x * 4 + (threadIdx.x >> 6),
+                      seg, nseg, nseg > 1 ? part + ((size_t)seg * gridDim.y + b) * nsino : nullptr);
+}
+
+// out[i] = sum_k part[k n + i], k ascending (deterministic).
+__global__ __launch_bounds__(256) void sum_parts_kernel(const double* __restrict__ part, float* __restrict__ out,
+                                                        size_t n, int nparts) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int k = 0; k < nparts; ++k) acc += part[(size_t)k * n + i];
+    out[i] = (float)acc;
+  }
 }
 
 
@@ -43,16 +59,21 @@ int max_tile_window(const Proj2DArgs& g, const ViewGeo* views_host) {
   return w;
 }
 
-// Bin records are staged in LDS with the sinogram value in their first
-
+// grid (tiles x, tiles y, batch * ngroup): view group k = blockIdx.z % ngroup
+// covers views [k vpg, (k + 1) vpg); with ngroup > 1 it writes FP64 partial
+// images to part[(k * batch + image) * nx * ny + ...] for sum_parts_kernel.
 template <int BASIS, bool USE_LDS>
-__global__ __launch_bounds__(256) void bwd_pix_kernel(Proj2DArgs g, const float* __restrict__ sino,
-                                                      float* __restrict__ img) {
+__global__ __launch_bounds__(256, 8) void bwd_pix_kernel(Proj2DArgs g, const float* __restrict__ sino,
+                                                      float* __restrict__ img, int ngroup, int vpg,
+                                                      double* __restrict__ part) {
   using Bin = typename Traits<BASIS>::Bin;
   __shared__ Bin lds[USE_LDS ? kViewChunk : 1][kMaxWindow];
   __shared__ int win[2 * kViewChunk];
-  bwd_pix_tile<BASIS, USE_LDS>(g, sino + (size_t)blockIdx.z * g.np * g.nu, img + (size_t)blockIdx.z * g.nx * g.ny,
-                               blockIdx.x, blockIdx.y, lds, win);
+  const int k = blockIdx.z % ngroup, b = blockIdx.z / ngroup, batch = gridDim.z / ngroup;
+  const size_t nimg = (size_t)g.nx * g.ny;
+  const int v0 = k * vpg, v1 = min(g.np, v0 + vpg);
+  bwd_pix_tile<BASIS, USE_LDS>(g, sino + (size_t)b * g.np * g.nu, img + b * nimg, blockIdx.x, blockIdx.y, lds, win,
+                               v0, v1, ngroup > 1 ? part + ((size_t)k * batch + b) * nimg : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -90,27 +111,41 @@ __global__ __launch_bounds__(256) void count_kernel(Proj2DArgs g, unsigned long 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, float* sino, int batch,
-                          hipStream_t s) {
+// nseg > 1 needs part with nseg * batch * np * nu doubles.
+hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, float* sino, int batch, int nseg,
+                          double* part, hipStream_t s) {
   const int waves = g.np * g.nchunk;
-  dim3 grid((waves + 3) / 4, batch), block(256);
-  if (basis == BASIS_B1) fwd_bin_kernel<BASIS_B1><<<grid, block, 0, s>>>(g, img, sino);
-  else if (basis == BASIS_B0) fwd_bin_kernel<BASIS_B0><<<grid, block, 0, s>>>(g, img, sino);
+  if (nseg < 1 || (nseg > 1 && !part)) return hipErrorInvalidValue;
+  dim3 grid((waves + 3) / 4, batch, nseg), block(256);
+  if (basis == BASIS_B1) fwd_bin_kernel<BASIS_B1><<<grid, block, 0, s>>>(g, img, sino, part);
+  else if (basis == BASIS_B0) fwd_bin_kernel<BASIS_B0><<<grid, block, 0, s>>>(g, img, sino, part);
   else return hipErrorInvalidValue;
+  if (nseg > 1) {
+    const size_t n = (size_t)batch * g.np * g.nu;
+    sum_parts_kernel<<<(unsigned)std::min<size_t>((n + 255) / 256, 4096), 256, 0, s>>>(part, sino, n, nseg);
+  }
   return hipGetLastError();
 }
 
+// ngroup > 1 needs part with ngroup * batch * nx * ny doubles.
 hipError_t launch_backward(int basis, const Proj2DArgs& g, bool use_lds, const float* sino, float* img, int batch,
-                           hipStream_t s) {
-  dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, batch), block(256);
+                           int ngroup, double* part, hipStream_t s) {
+  if (ngroup < 1 || (ngroup > 1 && !part)) return hipErrorInvalidValue;
+  // views per group, whole view chunks
+  const int vpg = ((g.np + ngroup - 1) / ngroup + kViewChunk - 1) / kViewChunk * kViewChunk;
+  dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, batch * ngroup), block(256);
   if (basis == BASIS_B1) {
-    if (use_lds) bwd_pix_kernel<BASIS_B1, true><<<grid, block, 0, s>>>(g, sino, img);
-    else bwd_pix_kernel<BASIS_B1, false><<<grid, block, 0, s>>>(g, sino, img);
+    if (use_lds) bwd_pix_kernel<BASIS_B1, true><<<grid, block, 0, s>>>(g, sino, img, ngroup, vpg, part);
+    else bwd_pix_kernel<BASIS_B1, false><<<grid, block, 0, s>>>(g, sino, img, ngroup, vpg, part);
   } else if (basis == BASIS_B0) {
-    if (use_lds) bwd_pix_kernel<BASIS_B0, true><<<grid, block, 0, s>>>(g, sino, img);
-    else bwd_pix_kernel<BASIS_B0, false><<<grid, block, 0, s>>>(g, sino, img);
+    if (use_lds) bwd_pix_kernel<BASIS_B0, true><<<grid, block, 0, s>>>(g, sino, img, ngroup, vpg, part);
+    else bwd_pix_kernel<BASIS_B0, false><<<grid, block, 0, s>>>(g, sino, img, ngroup, vpg, part);
   } else {
     return hipErrorInvalidValue;
+  }
+  if (ngroup > 1) {
+    const size_t n = (size_t)batch * g.nx * g.ny;
+    sum_parts_kernel<<<(unsigned)std::min<size_t>((n + 255) / 256, 4096), 256, 0, s>>>(part, img, n, ngroup);
   }
   return hipGetLastError();
 }

@@ -19,11 +19,13 @@ template <int BASIS> struct Traits;
 
 template <> struct Traits<BASIS_B1> {
   using Bin = B1Bin;
+  // (S(tb') - S(ta')) / (tb' - ta'); times scale(r) it is the footprint.
   __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
-    float sa = s_b1(ta, r.apb, r.b, r.a, r.k2, r.k4, r.cq, r.c2);
-    float sb = s_b1(tb, r.apb, r.b, r.a, r.k2, r.k4, r.cq, r.c2);
+    float sa = s_b1(ta, r.apb, r.b, r.a, r.k2, r.k4);
+    float sb = s_b1(tb, r.apb, r.b, r.a, r.k2, r.k4);
     return mean_over(sa, sb, ta, tb);
   }
+  __host__ __device__ static __forceinline__ float scale(const Bin& r) { return r.scale; }
 };
 
 template <> struct Traits<BASIS_B0> {
@@ -33,15 +35,20 @@ template <> struct Traits<BASIS_B0> {
     float sb = s_b0(tb, r.hpq, r.hmq, r.inv_p, r.inv_2pq);
     return mean_over(sa, sb, ta, tb);
   }
+  __host__ __device__ static __forceinline__ float scale(const Bin&) { return 1.0f; }
 };
 
 // ---------------------------------------------------------------------------
 // A: bin-driven gather.
 // ---------------------------------------------------------------------------
 // One wave: bins [64 chunk, 64 chunk + 64) of view v, wave = v * nchunk + chunk.
+// With nseg > 1 the wave covers only lines [seg L, seg L + L) of its walk
+// (L = ceil(lines / nseg)) and writes its FP64 partial sum, already scaled,
+// to part[v * nu + u]; otherwise it writes the finished value to sino.
 template <int BASIS>
 __device__ __forceinline__ void fwd_bin_wave(const Proj2DArgs& g, const float* __restrict__ img,
-                                             float* __restrict__ sino, int wave) {
+                                             float* __restrict__ sino, int wave, int seg = 0, int nseg = 1,
+                                             double* __restrict__ part = nullptr) {
   using T = Traits<BASIS>;
   const int lane = threadIdx.x & 63;
   const int v = wave / g.nchunk;
@@ -97,72 +104,128 @@ __device__ __forceinline__ void fwd_bin_wave(const Proj2DArgs& g, const float* _
     m_lo = (int)lo;
     m_hi = (int)hi;
   }
+  if (nseg > 1) {
+    const int len = (nmaj + nseg - 1) / nseg;
+    m_lo = max(m_lo, seg * len);
+    m_hi = min(m_hi, seg * len + len - 1);
+  }
 
-  // Pixel interval of line m, from the two pixel-set tests
-  //   q:  N_q + R_q >= 0  <=>  sq n >= -ra - qc0 - mq m
-  //   r:  N_r - R_r <  0  <=>  sr n <  rb - rc0 - mr m
-  // Each test with a nonzero slope along the walk bounds n from one side
-  // (affine in m, evaluated in FP64: B0's model value at the corner-range
-  // boundary is not small, so the decision must match an FP64 evaluation of
-  // the rule); a test with zero slope bounds the line index instead.  Exact
-  // integer ties are not special-cased: the corner rule itself is
-  // discontinuous there.
-  constexpr double kFar = 1e300;
-  double lq0 = -kFar, lq1 = 0.0, uq0 = kFar, uq1 = 0.0;
-  double lr0 = -kFar, lr1 = 0.0, ur0 = kFar, ur1 = 0.0;
-  if (sq > 0.0) {
-    lq0 = (-ra - qc0) * isq;
-    lq1 = -mq * isq;
-  } else if (sq < 0.0) {
-    uq0 = (-ra - qc0) * isq;
-    uq1 = -mq * isq;
-  } else if (mq > 0.0) {
-    m_lo = max(m_lo, (int)fmin(ceil((-ra - qc0) / mq), 1e9));
-  } else if (mq < 0.0) {
-    m_hi = min(m_hi, (int)fmax(floor((-ra - qc0) / mq), -1e9));
-  } else if (-ra - qc0 > 0.0) {
-    m_hi = -1;
-  }
-  if (sr > 0.0) {
-    ur0 = (rb - rc0) * isr;
-    ur1 = -mr * isr;
-  } else if (sr < 0.0) {
-    lr0 = (rb - rc0) * isr;
-    lr1 = -mr * isr;
-  } else if (mr > 0.0) {
-    m_hi = min(m_hi, (int)fmax(ceil((rb - rc0) / mr) - 1.0, -1e9));
-  } else if (mr < 0.0) {
-    m_lo = max(m_lo, (int)fmin(floor((rb - rc0) / mr) + 1.0, 1e9));
-  } else if (rb - rc0 <= 0.0) {
-    m_hi = -1;
-  }
-  // ta, tb at a line's first pixel in FP64: ta = ka N_u = an * n + am * m + a0
+  // ta, tb at pixel n of line m in FP64: ta = ka N_u = a_m + an n with
+  // a_m = a0 + am m carried from line to line.
   const double an = (double)rec.ka * sa, am = (double)rec.ka * ma, a0 = (double)rec.ka * ac0;
   const double bn = (double)rec.kb * sb, bm = (double)rec.kb * mb, b0 = (double)rec.kb * bc0;
-  const double lastn = (double)(nmin - 1);
-
+  double ta_m = __builtin_fma((double)m_lo, am, a0), tb_m = __builtin_fma((double)m_lo, bm, b0);
   double acc = 0.0;
-  for (int m = m_lo; m <= m_hi; ++m) {
-    const double dm = (double)m;
-    const double flo = fmax(fmax(ceil(fma(dm, lq1, lq0)), floor(fma(dm, lr1, lr0)) + 1.0), 0.0);
-    const double fhi = fmin(fmin(floor(fma(dm, uq1, uq0)), ceil(fma(dm, ur1, ur0)) - 1.0), lastn);
-    if (flo > fhi) continue;
+
+  // Image reads go through a buffer resource with 32-bit byte offsets (the
+  // image is under 4 GiB; reads past its end would return zero).
+  const __amdgpu_buffer_rsrc_t img_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(img), (short)0, (int)(4u * (unsigned)g.nx * (unsigned)g.ny), 0x00020000);
+  const unsigned bstep = 4u * (unsigned)smin;
+
+  // One line: pixels [flo, fhi] (flo <= fhi) of line m.
+  auto walk_line = [&](int m, float flo, float fhi) {
     const int n0 = (int)flo;
-    const double dn = flo;
-    float ta = (float)__builtin_fma(dn, an, __builtin_fma(dm, am, a0));
-    float tb = (float)__builtin_fma(dn, bn, __builtin_fma(dm, bm, b0));
-    int idx = m * smaj + n0 * smin;
-    const int end = idx + ((int)fhi - n0 + 1) * smin;
+    const double dn = (double)flo;
+    float ta = (float)__builtin_fma(dn, an, ta_m);
+    float tb = (float)__builtin_fma(dn, bn, tb_m);
+    unsigned off = 4u * (unsigned)(m * smaj + n0 * smin);
+    const unsigned end = off + (unsigned)((int)fhi - n0 + 1) * bstep;
     float accl = 0.0f;
     do {
-      accl = __builtin_fmaf(T::conv(rec, ta, tb), img[idx], accl);
-      idx += smin;
+      const float x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(img_rsrc, (int)off, 0, 0));
+      accl = __builtin_fmaf(T::conv(rec, ta, tb), x, accl);
+      off += bstep;
       ta += dta;
       tb += dtb;
-    } while (idx != end);
+    } while (off != end);
     acc += (double)accl;
+  };
+
+  if constexpr (BASIS == BASIS_B1) {
+    // Pixel interval of line m: [ceil(lo(m)), floor(hi(m))] from the two
+    // pixel-set tests (below), lo and hi affine in m, in FP32.  B1's
+    // footprint vanishes (as the cube of the distance) at the edge of the
+    // corner range, so a decision flipped by FP32 rounding there changes A
+    // by far less than FP32 noise.  sq and sr share a sign whenever the strip
+    // is within ~45 degrees of the walk direction, which the per-wave
+    // orientation guarantees; otherwise the lane walks whole lines (outside
+    // the corner range B1's footprint is zero).
+    float lo0 = 0.0f, lo1 = 0.0f, hi0 = (float)(nmin - 1), hi1 = 0.0f;
+    if (sq > 0.0 && sr > 0.0) {
+      lo0 = (float)((-ra - qc0) * isq);
+      lo1 = (float)(-mq * isq);
+      hi0 = (float)((rb - rc0) * isr);
+      hi1 = (float)(-mr * isr);
+    } else if (sq < 0.0 && sr < 0.0) {
+      hi0 = (float)((-ra - qc0) * isq);
+      hi1 = (float)(-mq * isq);
+      lo0 = (float)((rb - rc0) * isr);
+      lo1 = (float)(-mr * isr);
+    }
+    const float lastn = (float)(nmin - 1);
+    for (int m = m_lo; m <= m_hi; ++m) {
+      const float fm = (float)m;
+      const float flo = fmaxf(ceilf(__builtin_fmaf(fm, lo1, lo0)), 0.0f);
+      const float fhi = fminf(floorf(__builtin_fmaf(fm, hi1, hi0)), lastn);
+      if (flo <= fhi) walk_line(m, flo, fhi);
+      ta_m += am;
+      tb_m += bm;
+    }
+  } else {
+    // Pixel interval of line m, from the two pixel-set tests
+    //   q:  N_q + R_q >= 0  <=>  sq n >= -ra - qc0 - mq m
+    //   r:  N_r - R_r <  0  <=>  sr n <  rb - rc0 - mr m
+    // Each test with a nonzero slope along the walk bounds n from one side
+    // (affine in m, evaluated in FP64: B0's model value at the corner-range
+    // boundary is not small, so the decision must match an FP64 evaluation
+    // of the rule); a test with zero slope bounds the line index instead.
+    // Exact integer ties are not special-cased: the corner rule itself is
+    // discontinuous there.
+    constexpr double kFar = 1e300;
+    double lq0 = -kFar, lq1 = 0.0, uq0 = kFar, uq1 = 0.0;
+    double lr0 = -kFar, lr1 = 0.0, ur0 = kFar, ur1 = 0.0;
+    if (sq > 0.0) {
+      lq0 = (-ra - qc0) * isq;
+      lq1 = -mq * isq;
+    } else if (sq < 0.0) {
+      uq0 = (-ra - qc0) * isq;
+      uq1 = -mq * isq;
+    } else if (mq > 0.0) {
+      m_lo = max(m_lo, (int)fmin(ceil((-ra - qc0) / mq), 1e9));
+    } else if (mq < 0.0) {
+      m_hi = min(m_hi, (int)fmax(floor((-ra - qc0) / mq), -1e9));
+    } else if (-ra - qc0 > 0.0) {
+      m_hi = -1;
+    }
+    if (sr > 0.0) {
+      ur0 = (rb - rc0) * isr;
+      ur1 = -mr * isr;
+    } else if (sr < 0.0) {
+      lr0 = (rb - rc0) * isr;
+      lr1 = -mr * isr;
+    } else if (mr > 0.0) {
+      m_hi = min(m_hi, (int)fmax(ceil((rb - rc0) / mr) - 1.0, -1e9));
+    } else if (mr < 0.0) {
+      m_lo = max(m_lo, (int)fmin(floor((rb - rc0) / mr) + 1.0, 1e9));
+    } else if (rb - rc0 <= 0.0) {
+      m_hi = -1;
+    }
+    ta_m = __builtin_fma((double)m_lo, am, a0);
+    tb_m = __builtin_fma((double)m_lo, bm, b0);
+    const double lastn = (double)(nmin - 1);
+    for (int m = m_lo; m <= m_hi; ++m) {
+      const double dm = (double)m;
+      const double flo = fmax(fmax(ceil(fma(dm, lq1, lq0)), floor(fma(dm, lr1, lr0)) + 1.0), 0.0);
+      const double fhi = fmin(fmin(floor(fma(dm, uq1, uq0)), ceil(fma(dm, ur1, ur0)) - 1.0), lastn);
+      if (flo <= fhi) walk_line(m, (float)flo, (float)fhi);
+      ta_m += am;
+      tb_m += bm;
+    }
   }
-  sino[(size_t)v * g.nu + u] = (float)(acc * g.dx * g.dy);
+  const double val = acc * g.dx * g.dy * T::scale(rec);
+  if (part) part[(size_t)v * g.nu + u] = val;
+  else sino[(size_t)v * g.nu + u] = (float)val;
 }
 
 
@@ -191,23 +254,27 @@ __host__ __device__ inline void tile_window(const Proj2DArgs& g, const ViewGeo& 
   *n = hi >= li ? min(hi - li + 1, cap) : 0;
 }
 
-// Bin records are staged in LDS with the sinogram value in their first
-// padding word (B1Bin::pad0 / B0Bin::pad0).
-__device__ __forceinline__ void set_y(B1Bin& r, float y) { r.pad0 = y; }
-__device__ __forceinline__ void set_y(B0Bin& r, float y) { r.pad0 = y; }
-__device__ __forceinline__ float get_y(const B1Bin& r) { return r.pad0; }
-__device__ __forceinline__ float get_y(const B0Bin& r) { return r.pad0; }
+// A^T stages bin records in LDS with the sinogram value (times the footprint
+// scale) in place of the scale: stage() writes it, staged_y() reads it.
+__device__ __forceinline__ void stage(B1Bin& r, float y) { r.scale *= y; }
+__device__ __forceinline__ void stage(B0Bin& r, float y) { r.y = y; }
+__device__ __forceinline__ float staged_y(const B1Bin& r) { return r.scale; }
+__device__ __forceinline__ float staged_y(const B0Bin& r) { return r.y; }
 
 // USE_LDS: the host has checked that every tile's bin window fits in
 // kMaxWindow bins for every view (projector creation); otherwise records are
 // read from global memory (L2) directly.
 // One 16x16 pixel tile per 256-thread block.  `lds` holds kViewChunk x
 // kMaxWindow records (USE_LDS), `win` 2 x kViewChunk ints; every thread of
-// the block calls this (it synchronises the block).
+// the block calls this (it synchronises the block).  Views [v_begin, v_end)
+// are gathered; with `part` the tile writes its FP64 sums, already scaled,
+// to part[iy * nx + ix], otherwise the finished image values to img.
 template <int BASIS, bool USE_LDS>
 __device__ __forceinline__ void bwd_pix_tile(const Proj2DArgs& g, const float* __restrict__ sino,
                                              float* __restrict__ img, int tile_x, int tile_y,
-                                             typename Traits<BASIS>::Bin (*lds)[kMaxWindow], int* win) {
+                                             typename Traits<BASIS>::Bin (*lds)[kMaxWindow], int* win,
+                                             int v_begin = 0, int v_end = -1, double* __restrict__ part = nullptr) {
+  if (v_end < 0) v_end = g.np;
   using T = Traits<BASIS>;
   using Bin = typename T::Bin;
   static_assert(sizeof(Bin) % 16 == 0, "bin records are whole float4s");
@@ -223,8 +290,8 @@ __device__ __forceinline__ void bwd_pix_tile(const Proj2DArgs& g, const float* _
   const float last = (float)(g.nu - 1);
 
   double acc = 0.0;
-  for (int v0 = 0; v0 < g.np; v0 += kViewChunk) {
-    const int nv = min(kViewChunk, g.np - v0);
+  for (int v0 = v_begin; v0 < v_end; v0 += kViewChunk) {
+    const int nv = min(kViewChunk, v_end - v0);
     if (USE_LDS) {
       if ((int)threadIdx.x < nv) {
         int lo, n;
@@ -238,7 +305,7 @@ __device__ __forceinline__ void bwd_pix_tile(const Proj2DArgs& g, const float* _
         if (i < win_n[j]) {
           const size_t gi = (size_t)(v0 + j) * g.nu + win_lo[j] + i;
           Bin r = bins[gi];
-          set_y(r, sino[gi]);
+          stage(r, sino[gi]);
           lds[j][i] = r;
         }
       }
@@ -275,7 +342,7 @@ __device__ __forceinline__ void bwd_pix_tile(const Proj2DArgs& g, const float* _
           for (int k = 0; k < cnt; ++k) {
             const Bin rec = rp[k];
             const float nb = na + bef;
-            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), get_y(rec), accv);
+            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), staged_y(rec), accv);
             na = nb;
           }
         } else {
@@ -283,7 +350,7 @@ __device__ __forceinline__ void bwd_pix_tile(const Proj2DArgs& g, const float* _
           for (int k = 0; k < cnt; ++k) {
             const Bin rec = bins[base + k];
             const float nb = na + bef;
-            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), sino[base + k], accv);
+            accv = __builtin_fmaf(T::conv(rec, rec.ka * na, rec.kb * nb), sino[base + k] * T::scale(rec), accv);
             na = nb;
           }
         }
@@ -292,7 +359,11 @@ __device__ __forceinline__ void bwd_pix_tile(const Proj2DArgs& g, const float* _
     }
     if (USE_LDS) __syncthreads();
   }
-  if (valid) img[(size_t)iy * g.nx + ix] = (float)(acc * g.dx * g.dy);
+  if (valid) {
+    const double val = acc * g.dx * g.dy;
+    if (part) part[(size_t)iy * g.nx + ix] = val;
+    else img[(size_t)iy * g.nx + ix] = (float)val;
+  }
 }
 
 

@@ -1,0 +1,115 @@
+"""Timings of the workloads beside the headline pair (one JSON line each).
+
+    python -m sgd_amd.measure batch   # 256 uneven fan slices: per-slice launches,
+                                      # persistent launch, autotuned, graph replay
+    python -m sgd_amd.measure cone    # cone-beam B1 / SF-TR / SF-TT, 256^3 volume,
+                                      # 360 views, 512 x 384 detector
+    python -m sgd_amd.measure unet    # U-Net (4 levels, 32 channels) inference and
+                                      # training step, batch 8 at 512 x 512
+
+Times are HIP-event averages on the launch stream with inputs resident in
+device memory.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+
+def _time(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def batch(reps=3, tune_cache=None):
+    from .batch import SliceBatch, make_projectors, uneven_slices
+    t0 = time.perf_counter()
+    B = SliceBatch(make_projectors(uneven_slices(256, seed=0)))
+    build_s = time.perf_counter() - t0
+    img = B.empty_images().uniform_()
+    sino = B.empty_sinos().uniform_()
+    out_s, out_i = B.empty_sinos(), B.empty_images()
+    rec = dict(workload="256 uneven fan slices (B0/B1/SF, 128^2..768^2 pixels, 180..1440 views, 1.5 bins per "
+                        "pixel edge), uniform random inputs",
+               pixels=B.img_off[-1], bins=B.sino_off[-1], build_s=build_s)
+    rec["per_slice_ms"] = dict(A=_time(lambda: B.forward_per_slice(img, out_s), reps),
+                               AT=_time(lambda: B.backward_per_slice(sino, out_i), reps))
+    ref_s, ref_i = out_s.clone(), out_i.clone()
+    B.set_config(8, 4, 0)
+    rec["persistent_default_ms"] = dict(A=_time(lambda: B.forward(img, out_s), reps),
+                                        AT=_time(lambda: B.backward(sino, out_i), reps))
+    rec["persistent_equals_per_slice"] = bool(torch.equal(out_s, ref_s)) and bool(torch.equal(out_i, ref_i))
+    t = B.tune(img, sino, cache_path=tune_cache, reps=2, force=True)
+    rec["tuned"] = {k: t[k] for k in ("fwd_blocks_per_cu", "bwd_blocks_per_cu", "order", "fwd_ms", "bwd_ms")}
+    rec["tune_trials"] = t["trials"]
+    s = torch.cuda.Stream()  # graphs are captured on a created stream, not the null stream
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        rec["graph_ms"] = dict(A=_time(lambda: B.forward(img, out_s, graph=True), reps),
+                               AT=_time(lambda: B.backward(sino, out_i, graph=True), reps))
+    s.synchronize()
+    rec["graph_equals_per_slice"] = bool(torch.equal(out_s, ref_s)) and bool(torch.equal(out_i, ref_i))
+    return rec
+
+
+def cone(reps=3):
+    from .torch_ops import Projector
+    ang = 2 * np.pi * np.arange(360) / 360
+    recs = []
+    for model in ("B1", "SF-TR", "SF-TT"):
+        P = Projector.cone_beam(256, 256, 256, ang, 512, 384, 1.0, 1.0, 1.0, 1.0, 1.0, 1000.0, 500.0, model=model)
+        x = torch.rand(P.img_shape, device=P.device)
+        y = torch.rand(P.sino_shape, device=P.device)
+        recs.append(dict(workload="cone beam %s, 256^3 voxels (1 mm), 360 views, 512 x 384 detector (1 mm), "
+                                  "lsd 1000 / lso 500 mm" % model, model=model,
+                         A_ms=_time(lambda: P.forward(x), reps), AT_ms=_time(lambda: P.backward(y), reps)))
+    return recs
+
+
+def unet(reps=5, batch_size=8, size=512):
+    from .unet import UNet, layer_specs
+    net = UNet(4, 32, seed=0)
+    x = torch.rand(batch_size, size, size, device="cuda")
+    t = x * 0.9
+    for _ in range(2):
+        net.train_step(x, t, lr=1e-4)
+    flop = 0.0
+    for name, ci, co, k in layer_specs(4, 32):
+        level = int(name[3]) if name != "out" else 0
+        flop += 2.0 * k * k * ci * co * batch_size * (size >> level) ** 2
+    fwd = _time(lambda: net.forward(x), reps)
+    train = _time(lambda: net.train_step(x, t, lr=1e-4), reps)
+    return dict(workload="U-Net 4 levels / 32 channels, bf16 operands, FP32 accumulation, batch %d at %dx%d"
+                % (batch_size, size, size), conv_gflop_forward=flop / 1e9,
+                forward_ms=fwd, train_step_ms=train,
+                forward_tflops=flop / (fwd * 1e-3) / 1e12, train_tflops=3 * flop / (train * 1e-3) / 1e12)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("what", choices=["batch", "cone", "unet"])
+    ap.add_argument("--tune-cache", default=None)
+    a = ap.parse_args(argv)
+    if a.what == "batch":
+        recs = [batch(tune_cache=a.tune_cache)]
+    elif a.what == "cone":
+        recs = cone()
+    else:
+        recs = [unet()]
+    for r in recs:
+        print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

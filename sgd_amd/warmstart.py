@@ -15,9 +15,17 @@ b = -ln(max(counts, 1) / N0) / mu.
             restarts from the SIRT(5) image if its residual rises (the plain
             method stays the judge: the stopping rule is the same).
 
-`python -m sgd_amd.warmstart --help` runs the whole experiment and writes a
-JSON report.  No reference counterpart (the reference's optimizer/asd_pocs.cu
-is empty).
+Tolerance.  The Poisson noise of this workload is 0.7 % of ||b|| (relative
+||b - b_clean|| / ||b|| of 0.0068..0.009 on seeded phantoms), and with 90 x
+768 rays against 512^2 unknowns no iterate of these methods comes near 1e-3:
+unregularised SIRT without the positivity constraint is at 2.3e-3 after 3000
+iterations, with positivity at 3.6e-3, and ASD-POCS stalls at 4e-3..8e-3
+depending on its TV weight.  The default tolerance is therefore 1e-2, just
+above the noise level (the discrepancy principle); --tol sets another.
+
+`python -m sgd_amd.warmstart --help` runs the experiment (or one phase of it)
+and writes a JSON report.  No reference counterpart (the reference's
+optimizer/asd_pocs.cu is empty).
 """
 import argparse
 import json
@@ -34,7 +42,11 @@ from .unet import UNet
 GEOMETRY = dict(nx=512, ny=512, dx=0.6, dy=0.6, nu=768, du=0.8, lsd=1000.0, lso=500.0, views=90)
 MU = 0.02
 N0 = 5e4
-ASD = dict(alpha=0.2, r_max=0.95, ng=20, beta=1.0, beta_red=0.995, alpha_red=0.95)
+# ASD-POCS controls: TV step fraction alpha, its reduction alpha_red when the
+# TV change exceeds r_max times the data change, ng TV steps per iteration,
+# data-step relaxation beta reduced by beta_red per iteration.  With these the
+# plain method reaches 1e-2 in 170..410 iterations on this workload.
+ASD = dict(alpha=0.02, r_max=0.5, ng=20, beta=1.0, beta_red=0.999, alpha_red=0.95)
 
 
 def projector(geom=GEOMETRY):
@@ -134,58 +146,79 @@ def evaluate(P, net, b, tol=1e-3, max_iters=3000, batch=64, sirt_iters=5, log=pr
     return res
 
 
+def save_weights(net, path):
+    p = net.parameters_numpy()
+    np.savez(path, **{k + "_W": v[0] for k, v in p.items()}, **{k + "_b": v[1] for k, v in p.items()})
+
+
+def load_weights(path, levels=4, c0=32):
+    from .unet import layer_specs
+    with np.load(path) as z:
+        params = {name: (z[name + "_W"], z[name + "_b"]) for name, _, _, _ in layer_specs(levels, c0)}
+    return UNet(levels, c0, params=params)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--phase", choices=["all", "train", "eval"], default="all",
+                    help="train: generate pairs, train, save --weights; eval: load --weights, evaluate")
+    ap.add_argument("--weights", default=None, help="weights file (.npz) written by train / read by eval")
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--test", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=1e-2)
     ap.add_argument("--max-iters", type=int, default=3000)
     ap.add_argument("--asd-iters", type=int, default=200)
     ap.add_argument("--gen-batch", type=int, default=64)
     ap.add_argument("--out", default="warmstart_report.json")
-    ap.add_argument("--save-weights", default=None)
     a = ap.parse_args(argv)
+    if a.phase == "eval" and not a.weights:
+        ap.error("--phase eval needs --weights")
 
     def log(msg):
         print(msg, flush=True)
 
     P = projector()
-    t0 = time.perf_counter()
-    xin, xt, _ = make_pairs(P, list(range(a.pairs)), batch=a.gen_batch, asd_iters=a.asd_iters, log=log)
-    t_gen = time.perf_counter() - t0
-    net = UNet(4, 32, seed=0)
-    t0 = time.perf_counter()
-    losses = train(net, xin, xt, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
-    t_train = time.perf_counter() - t0
-    if a.save_weights:
-        np.savez(a.save_weights, **{k + "_W": v[0] for k, v in net.parameters_numpy().items()},
-                 **{k + "_b": v[1] for k, v in net.parameters_numpy().items()})
-    del xin, xt
-    seeds = list(range(10 ** 6, 10 ** 6 + a.test))
-    w = SirtWeights(P)
-    x = torch.cat([phantoms(seeds[k:k + 64], P.nx, P.ny, P.geom["dx"], P.geom["dy"], P.device)
-                   for k in range(0, len(seeds), 64)])
-    b = torch.cat([noisy_sinograms(P, x[k:k + 64], seed=seeds[k] + 7919) for k in range(0, len(seeds), 64)])
-    del x, w
-    ev = evaluate(P, net, b, tol=a.tol, max_iters=a.max_iters, log=log)
-    pi, wi = np.array(ev["plain_iters"]), np.array(ev["warm_iters"])
     report = dict(workload=dict(GEOMETRY, mu=MU, n0=N0, asd=ASD, pairs=a.pairs, test=a.test, tol=a.tol,
-                                max_iters=a.max_iters, asd_iters_target=a.asd_iters, epochs=a.epochs),
-                  seconds=dict(generate=t_gen, train=t_train, plain=ev["plain_s"], warm=ev["warm_s"]),
-                  iterations=dict(plain_mean=float(pi.mean()), warm_mean=float(wi.mean()),
-                                  plain_median=float(np.median(pi)), warm_median=float(np.median(wi)),
-                                  plain_reached=int((pi < a.max_iters).sum()),
-                                  warm_reached=int((wi < a.max_iters).sum())),
-                  speedup_iterations=float(pi.mean() / max(wi.mean(), 1e-9)),
-                  speedup_time=float(ev["plain_s"] / max(ev["warm_s"], 1e-9)),
-                  fell_back=ev["fell_back"], train_loss=losses[-5:])
+                                max_iters=a.max_iters, asd_iters_target=a.asd_iters, epochs=a.epochs,
+                                batch=a.batch, lr=a.lr),
+                  phase=a.phase, seconds={})
+    if a.phase in ("all", "train"):
+        t0 = time.perf_counter()
+        xin, xt, _ = make_pairs(P, list(range(a.pairs)), batch=a.gen_batch, asd_iters=a.asd_iters, log=log)
+        report["seconds"]["generate"] = time.perf_counter() - t0
+        net = UNet(4, 32, seed=0)
+        t0 = time.perf_counter()
+        losses = train(net, xin, xt, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+        report["seconds"]["train"] = time.perf_counter() - t0
+        report["train_loss"] = losses[-5:]
+        if a.weights:
+            save_weights(net, a.weights)
+        del xin, xt
+    else:
+        net = load_weights(a.weights)
+    if a.phase in ("all", "eval"):
+        seeds = list(range(10 ** 6, 10 ** 6 + a.test))
+        x = torch.cat([phantoms(seeds[k:k + 64], P.nx, P.ny, P.geom["dx"], P.geom["dy"], P.device)
+                       for k in range(0, len(seeds), 64)])
+        b = torch.cat([noisy_sinograms(P, x[k:k + 64], seed=seeds[k] + 7919) for k in range(0, len(seeds), 64)])
+        del x
+        ev = evaluate(P, net, b, tol=a.tol, max_iters=a.max_iters, log=log)
+        pi, wi = np.array(ev["plain_iters"]), np.array(ev["warm_iters"])
+        report["seconds"].update(plain=ev["plain_s"], warm=ev["warm_s"])
+        report.update(iterations=dict(plain_mean=float(pi.mean()), warm_mean=float(wi.mean()),
+                                      plain_median=float(np.median(pi)), warm_median=float(np.median(wi)),
+                                      plain_reached=int((pi < a.max_iters).sum()),
+                                      warm_reached=int((wi < a.max_iters).sum())),
+                      speedup_iterations=float(pi.mean() / max(wi.mean(), 1e-9)),
+                      speedup_time=float(ev["plain_s"] / max(ev["warm_s"], 1e-9)),
+                      fell_back=ev["fell_back"])
     with open(a.out, "w") as f:
         json.dump(report, f, indent=1)
-    log(json.dumps({k: report[k] for k in ("seconds", "iterations", "speedup_iterations", "speedup_time",
-                                           "fell_back")}))
+    log(json.dumps({k: report[k] for k in ("phase", "seconds", "iterations", "speedup_iterations", "speedup_time",
+                                           "fell_back") if k in report}))
     return report
 
 

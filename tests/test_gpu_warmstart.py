@@ -5,7 +5,7 @@
     ASD-POCS so that it reaches the tolerance in fewer iterations than from
     zero, on phantoms it was not trained on, with the fallback available.
 The full-size experiment (512^2, 90 views, 4096 pairs, 256 test phantoms,
-tolerance 1e-3) is `python -m sgd_amd.warmstart`.
+tolerance 1e-2, the noise level) is `python -m sgd_amd.warmstart`.
 """
 import numpy as np
 import pytest
