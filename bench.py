@@ -30,6 +30,14 @@ DU = 0.8
 LSD = 1000.0
 LSO = 500.0
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per B1 call at this workload, summed over the call's kernels
+# (the projector kernel and the partial-sum kernel), from rocprofv3 --pmc
+# FETCH_SIZE then WRITE_SIZE (KiB per dispatch; FETCH_SIZE scaled by 2.00 and
+# WRITE_SIZE by 1.00, the factors a 256 MiB device copy gives, --copy-calib):
+#   A:   fwd_bin 15231 x 2 + 69120, sum_parts 34571 x 2 + 2160 KiB
+#   A^T: bwd_pix 73487 x 2 + 32768, sum_parts 16395 x 2 + 1024 KiB
+# (profiles/b1_fanbeam_512x720x768.md).
+TRAFFIC_BYTES_B1 = {"A": 170884 * 1024.0, "A^T": 213556 * 1024.0}
 METRIC = "A+Aᵀ pairs/s at 512²/720v (%HBM peak)"
 
 
@@ -170,7 +178,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_B1[dom] if args.basis == "B1" else None,
             "kernel": dom,
             "bytes_per_launch": bytes_per_launch,
         },

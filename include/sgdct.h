@@ -139,7 +139,7 @@ int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int ba
  * in a fixed order (results are deterministic, independent of the batch size,
  * and agree across splits to FP64 rounding).  0 = automatic: chosen at
  * creation from one image's launch size (a launch of few waves splits, up to
- * 16 ways); 1..16 = fixed.  The partial-sum workspace (up to 4 images per
+ * 16 ways); 1..16 = fixed.  The partial-sum workspace (up to 2 images per
  * launch; larger batches take several launches) is allocated at creation and
  * by this call, never inside a projection.  Returns SGDCT_OK, SGDCT_EINVAL or
  * SGDCT_ENOMEM. */

@@ -200,7 +200,7 @@ int auto_bwd_split(const sgdct_projector* h);
 int ensure_part(sgdct_projector* h, size_t doubles);
 size_t part_need(const sgdct_projector* h);
 // images per launch the partial-sum workspace holds
-constexpr int kPartImages = 4;
+constexpr int kPartImages = 2;
 
 // Per-(view, bin) constants, FP64 on the host, stored FP32 like the
 // reference's per-bin vectors (layers/A_B1.cu:429-478, A_B0.cu:338-388).
@@ -422,7 +422,7 @@ int cone_backward(sgdct_projector* h, const float* proj, float* vol, int batch, 
 
 // Launch-size targets of the automatic splits (one image): enough waves for
 // every SIMD to hold several (A's per-wave work is uneven, A^T's is even).
-constexpr long long kFwdWaveTarget = 65536, kBwdWaveTarget = 32768;
+constexpr long long kFwdWaveTarget = 131072, kBwdWaveTarget = 65536;
 constexpr int kMaxSplit = 16;
 
 int auto_fwd_split(const sgdct_projector* h) {

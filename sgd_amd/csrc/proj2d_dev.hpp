@@ -110,11 +110,11 @@ __device__ __forceinline__ void fwd_bin_wave(const Proj2DArgs& g, const float* _
     m_hi = min(m_hi, seg * len + len - 1);
   }
 
-  // ta, tb at pixel n of line m in FP64: ta = ka N_u = a_m + an n with
-  // a_m = a0 + am m carried from line to line.
+  // ta, tb at pixel n of line m in FP64: ta = ka N_u = (a0 + am m) + an n,
+  // formed per line from m alone, so that a line's values do not depend on
+  // where the walk (or its segment) started.
   const double an = (double)rec.ka * sa, am = (double)rec.ka * ma, a0 = (double)rec.ka * ac0;
   const double bn = (double)rec.kb * sb, bm = (double)rec.kb * mb, b0 = (double)rec.kb * bc0;
-  double ta_m = __builtin_fma((double)m_lo, am, a0), tb_m = __builtin_fma((double)m_lo, bm, b0);
   double acc = 0.0;
 
   // Image reads go through a buffer resource with 32-bit byte offsets (the
@@ -126,9 +126,9 @@ __device__ __forceinline__ void fwd_bin_wave(const Proj2DArgs& g, const float* _
   // One line: pixels [flo, fhi] (flo <= fhi) of line m.
   auto walk_line = [&](int m, float flo, float fhi) {
     const int n0 = (int)flo;
-    const double dn = (double)flo;
-    float ta = (float)__builtin_fma(dn, an, ta_m);
-    float tb = (float)__builtin_fma(dn, bn, tb_m);
+    const double dn = (double)flo, dm = (double)m;
+    float ta = (float)__builtin_fma(dn, an, __builtin_fma(dm, am, a0));
+    float tb = (float)__builtin_fma(dn, bn, __builtin_fma(dm, bm, b0));
     unsigned off = 4u * (unsigned)(m * smaj + n0 * smin);
     const unsigned end = off + (unsigned)((int)fhi - n0 + 1) * bstep;
     float accl = 0.0f;
@@ -169,8 +169,6 @@ __device__ __forceinline__ void fwd_bin_wave(const Proj2DArgs& g, const float* _
       const float flo = fmaxf(ceilf(__builtin_fmaf(fm, lo1, lo0)), 0.0f);
       const float fhi = fminf(floorf(__builtin_fmaf(fm, hi1, hi0)), lastn);
       if (flo <= fhi) walk_line(m, flo, fhi);
-      ta_m += am;
-      tb_m += bm;
     }
   } else {
     // Pixel interval of line m, from the two pixel-set tests
@@ -211,16 +209,12 @@ __device__ __forceinline__ void fwd_bin_wave(const Proj2DArgs& g, const float* _
     } else if (rb - rc0 <= 0.0) {
       m_hi = -1;
     }
-    ta_m = __builtin_fma((double)m_lo, am, a0);
-    tb_m = __builtin_fma((double)m_lo, bm, b0);
     const double lastn = (double)(nmin - 1);
     for (int m = m_lo; m <= m_hi; ++m) {
       const double dm = (double)m;
       const double flo = fmax(fmax(ceil(fma(dm, lq1, lq0)), floor(fma(dm, lr1, lr0)) + 1.0), 0.0);
       const double fhi = fmin(fmin(floor(fma(dm, uq1, uq0)), ceil(fma(dm, ur1, ur0)) - 1.0), lastn);
       if (flo <= fhi) walk_line(m, (float)flo, (float)fhi);
-      ta_m += am;
-      tb_m += bm;
     }
   }
   const double val = acc * g.dx * g.dy * T::scale(rec);

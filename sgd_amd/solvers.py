@@ -168,42 +168,56 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
                    r_max=0.95, alpha_red=0.95, eps_data=0.0, tv_eps=1e-8, weights=None, check_every=10):
     """ASD-POCS (Sidky & Pan 2008) on a batch of sinograms b (B, np, nu) with
     per-image TV step control and stopping.  An image stops at the first
-    iteration whose entering residual ||A x - b|| / ||b|| is <= tol (its
-    iteration count is the number of data steps taken before that).  With
+    iteration whose entering residual ||A x - b|| / ||b|| is <= tol (a number,
+    or one per image as a (B,) tensor; its iteration count is the number of
+    data steps taken before that).  With
     `fallback` (B, ny, nx), an image whose residual rises between two
     iterations of its warm-started run restarts once from its fallback image.
-    The data step's relaxation beta follows one schedule for the batch."""
+    The data step's relaxation beta follows one schedule for the batch.
+
+    Stopped images leave the working batch at the next check (every
+    `check_every` iterations), so the work per iteration follows the number
+    of images still running.  Every image's iterates are the ones it would
+    have alone: the projector gives each image of a batch the result it gets
+    by itself, and all other steps are per image."""
     t0 = time.perf_counter()
     L = _lib.lib()
     b = b.contiguous().float()
     nb = b.shape[0]
     w = weights or SirtWeights(P)
     dev = b.device
-    x = torch.zeros((nb,) + tuple(P.img_shape), device=dev) if x0 is None else \
+    x_out = torch.zeros((nb,) + tuple(P.img_shape), device=dev) if x0 is None else \
         x0.clone().contiguous().float().clamp_min(0)
-    bn = _norms(b)
-    active = torch.ones(nb, dtype=torch.bool, device=dev)
-    fell = torch.zeros(nb, dtype=torch.bool, device=dev)
     iters = torch.full((nb,), niter, dtype=torch.int64, device=dev)
     last_res = torch.full((nb,), float("nan"), dtype=torch.float64, device=dev)
+    fell = torch.zeros(nb, dtype=torch.bool, device=dev)
+    # working set: the images still running, idx = their positions in the batch
+    idx = torch.arange(nb, device=dev)
+    x = x_out.clone()
+    bw = b
+    bn = _norms(b)
+    fb = fallback.float().contiguous() if fallback is not None else None
+    fellw = fell.clone()
+    active = torch.ones(nb, dtype=torch.bool, device=dev)
     prev_res = None
     dtvg = torch.full((nb,), float("nan"), dtype=torch.float64, device=dev)
+    tolw = None if tol is None else torch.as_tensor(tol, dtype=torch.float64, device=dev).expand(nb).clone()
     g = torch.empty_like(x)
     coef = torch.empty(nb, device=dev)
     per = x[0].numel()
     for k in range(niter):
         x_prev = x.clone()
-        ax = sirt_step(P, w, x, b, beta, positivity=True)
-        res = _norms(ax - b) / bn
-        last_res = torch.where(active, res, last_res)
-        if tol is not None:
-            done = active & (res <= tol)
-            iters = torch.where(done, torch.full_like(iters, k), iters)
+        ax = sirt_step(P, w, x, bw, beta, positivity=True)
+        res = _norms(ax - bw) / bn
+        last_res[idx] = torch.where(active, res, last_res[idx])
+        if tolw is not None:
+            done = active & (res <= tolw)
+            iters[idx] = torch.where(done, torch.full_like(idx, k), iters[idx])
             active = active & ~done
-        if fallback is not None and prev_res is not None:
-            rise = active & ~fell & (res > prev_res)
-            x = torch.where(rise[:, None, None], fallback.float(), x)
-            fell = fell | rise
+        if fb is not None and prev_res is not None:
+            rise = active & ~fellw & (res > prev_res)
+            x = torch.where(rise[:, None, None], fb, x)
+            fellw = fellw | rise
             dtvg = torch.where(rise, torch.full_like(dtvg, float("nan")), dtvg)
         prev_res = res
         x = torch.where(active[:, None, None], x, x_prev).contiguous()
@@ -219,10 +233,24 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
         dg = _norms(x - x_tv0)
         dtvg = torch.where((dg > r_max * dp) & (res > eps_data), dtvg * alpha_red, dtvg)
         beta *= beta_red
-        if tol is not None and (k + 1) % check_every == 0 and not bool(active.any()):
-            break
+        if tol is not None and (k + 1) % check_every == 0:
+            n_act = int(active.sum())
+            if n_act == 0:
+                break
+            if n_act < idx.numel():
+                x_out[idx] = x
+                fell[idx] = fellw
+                keep = active.nonzero().squeeze(1)
+                idx, x, bw, bn = idx[keep], x[keep].contiguous(), bw[keep].contiguous(), bn[keep]
+                dtvg, prev_res, fellw, active = dtvg[keep], prev_res[keep], fellw[keep], active[keep]
+                tolw = tolw[keep]
+                fb = fb[keep].contiguous() if fb is not None else None
+                g = torch.empty_like(x)
+                coef = torch.empty(x.shape[0], device=dev)
+    x_out[idx] = x
+    fell[idx] = fellw
     torch.cuda.synchronize()
-    return AsdPocsBatchResult(x, iters.cpu().numpy(), last_res.cpu().numpy(), fell.cpu().numpy(),
+    return AsdPocsBatchResult(x_out, iters.cpu().numpy(), last_res.cpu().numpy(), fell.cpu().numpy(),
                               time.perf_counter() - t0)
 
 
