@@ -127,13 +127,13 @@ void sgdct_destroy(sgdct_projector *h);
  * (cone beam: volume (nz, ny, nx) and projections (np, nv, nu)), both device
  * pointers; `batch` images / sinograms back to back; `stream` is
  * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.  Cone-beam
- * handles, and B0 / B1 handles on launches they split (below), keep per-handle
+ * handles, and B0 / B1 / SF handles on launches they split (below), keep per-handle
  * workspace: calls on one handle must not overlap (issue them on one stream).
  * Returns SGDCT_OK or a negative code. */
 int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
 
-/* B0 / B1 launch shape (no reference counterpart).  A splits every bin's
+/* B0 / B1 / SF launch shape (no reference counterpart).  A splits every bin's
  * pixel walk into `fwd_split` line segments and A^T the views into
  * `bwd_split` groups, each writing FP64 partial sums that a second kernel adds
  * in a fixed order (results are deterministic, independent of the batch size,

@@ -22,8 +22,10 @@ hipError_t launch_backward(int basis, const Proj2DArgs& g, bool use_lds, const f
                            int ngroup, double* part, hipStream_t s);
 int max_tile_window(const Proj2DArgs& g, const ViewGeo* views_host);
 hipError_t launch_count(const Proj2DArgs& g, unsigned long long* out, hipStream_t s);
-hipError_t launch_sf_forward(const Proj2DArgs& g, const float* img, float* sino, int batch, hipStream_t s);
-hipError_t launch_sf_backward(const Proj2DArgs& g, const float* sino, float* img, int batch, hipStream_t s);
+hipError_t launch_sf_forward(const Proj2DArgs& g, const float* img, float* sino, int batch, int nseg, double* part,
+                             hipStream_t s);
+hipError_t launch_sf_backward(const Proj2DArgs& g, const float* sino, float* img, int batch, int ngroup,
+                              double* part, hipStream_t s);
 void hex_bin_constants(int basis, const ViewGeo& v, int nu, double dt, void* out);
 size_t hex_bin_size(int basis);
 hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const float* img, float* sino, int batch,
@@ -125,7 +127,7 @@ struct sgdct_projector {
   unsigned long long* d_count = nullptr;
   long long evals = -1;
   bool bwd_lds = false;  // A^T stages bin windows in LDS (B0 / B1)
-  // B0 / B1: A splits each bin's walk into line segments and A^T the views
+  // B0 / B1 / SF: A splits each bin's walk into line segments and A^T the views
   // into groups; fwd_auto / bwd_auto are chosen at creation from one image's
   // launch size, fwd_split / bwd_split (0 = automatic) override them.  The
   // FP64 partial sums go to d_part (part_cap doubles, allocated at creation
@@ -325,8 +327,8 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   a.h = h->h;
   a.nchunk = (nu + 63) / 64;
   // 2 bins of slack for FP64 contraction differences between host and device
-  if (basis == BASIS_B0 || basis == BASIS_B1) {
-    h->bwd_lds = max_tile_window(a, views.data()) <= kMaxWindow - 2;
+  if (basis == BASIS_B0 || basis == BASIS_B1) h->bwd_lds = max_tile_window(a, views.data()) <= kMaxWindow - 2;
+  if (basis == BASIS_B0 || basis == BASIS_B1 || basis == BASIS_SF) {
     h->fwd_auto = auto_fwd_split(h);
     h->bwd_auto = auto_bwd_split(h);
     if (ensure_part(h, kPartImages * part_need(h)) != SGDCT_OK) {
@@ -456,22 +458,35 @@ size_t part_need(const sgdct_projector* h) {
   return std::max(fs > 1 ? (size_t)fs * h->np * h->nu : 0, bs > 1 ? (size_t)bs * h->nx * h->ny : 0);
 }
 
+bool splits_launches(int basis) { return basis == BASIS_B0 || basis == BASIS_B1 || basis == BASIS_SF; }
+
+hipError_t launch_fwd2d(const sgdct_projector* h, const float* img, float* sino, int batch, int nseg, double* part,
+                        hipStream_t s) {
+  if (h->basis == BASIS_SF) return launch_sf_forward(h->args, img, sino, batch, nseg, part, s);
+  return launch_forward(h->basis, h->args, img, sino, batch, nseg, part, s);
+}
+
+hipError_t launch_bwd2d(const sgdct_projector* h, const float* sino, float* img, int batch, int ngroup, double* part,
+                        hipStream_t s) {
+  if (h->basis == BASIS_SF) return launch_sf_backward(h->args, sino, img, batch, ngroup, part, s);
+  return launch_backward(h->basis, h->args, h->bwd_lds, sino, img, batch, ngroup, part, s);
+}
+
 int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
   if (is_cone(h->basis)) return cone_forward(h, img, sino, batch, s);
   hipError_t e = hipSuccess;
-  if (is_hex(h->basis)) e = launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, s);
-  else if (h->basis == BASIS_SF) e = launch_sf_forward(h->args, img, sino, batch, s);
-  else {
+  if (is_hex(h->basis)) {
+    e = launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, s);
+  } else {
     const int nseg = h->fwd_split ? h->fwd_split : h->fwd_auto;
     const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)nseg * ns;
     if (nseg == 1) {
-      e = launch_forward(h->basis, h->args, img, sino, batch, 1, nullptr, s);
+      e = launch_fwd2d(h, img, sino, batch, 1, nullptr, s);
     } else {
       if (h->part_cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
       const int bc = (int)std::min<size_t>(batch, h->part_cap / per);
       for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
-        e = launch_forward(h->basis, h->args, img + b0 * ni, sino + b0 * ns, std::min(bc, batch - b0), nseg,
-                           h->d_part, s);
+        e = launch_fwd2d(h, img + b0 * ni, sino + b0 * ns, std::min(bc, batch - b0), nseg, h->d_part, s);
     }
   }
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "forward launch");
@@ -480,19 +495,18 @@ int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hi
 int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
   if (is_cone(h->basis)) return cone_backward(h, sino, img, batch, s);
   hipError_t e = hipSuccess;
-  if (is_hex(h->basis)) e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
-  else if (h->basis == BASIS_SF) e = launch_sf_backward(h->args, sino, img, batch, s);
-  else {
+  if (is_hex(h->basis)) {
+    e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
+  } else {
     const int ng = h->bwd_split ? h->bwd_split : h->bwd_auto;
     const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)ng * ni;
     if (ng == 1) {
-      e = launch_backward(h->basis, h->args, h->bwd_lds, sino, img, batch, 1, nullptr, s);
+      e = launch_bwd2d(h, sino, img, batch, 1, nullptr, s);
     } else {
       if (h->part_cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
       const int bc = (int)std::min<size_t>(batch, h->part_cap / per);
       for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
-        e = launch_backward(h->basis, h->args, h->bwd_lds, sino + b0 * ns, img + b0 * ni, std::min(bc, batch - b0),
-                            ng, h->d_part, s);
+        e = launch_bwd2d(h, sino + b0 * ns, img + b0 * ni, std::min(bc, batch - b0), ng, h->d_part, s);
     }
   }
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "backward launch");
@@ -1020,7 +1034,7 @@ int sgdct_batch_graph_launch(sgdct_batch* b, int direction, const float* in, flo
 int sgdct_split_config(sgdct_projector* h, int fwd_split, int bwd_split) {
   if (!h || fwd_split < 0 || fwd_split > kMaxSplit || bwd_split < 0 || bwd_split > kMaxSplit)
     return fail(SGDCT_EINVAL, "split_config: splits must be 0 (automatic) or 1..16");
-  if (h->basis != BASIS_B0 && h->basis != BASIS_B1) return fail(SGDCT_EINVAL, "split_config: B0 / B1 handles only");
+  if (!splits_launches(h->basis)) return fail(SGDCT_EINVAL, "split_config: B0 / B1 / SF handles only");
   const int fs = h->fwd_split, bs = h->bwd_split;
   h->fwd_split = fwd_split;
   h->bwd_split = bwd_split;

@@ -111,6 +111,11 @@ __global__ __launch_bounds__(256) void count_kernel(Proj2DArgs g, unsigned long 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_sum_parts(const double* part, float* out, size_t n, int nparts, hipStream_t s) {
+  sum_parts_kernel<<<(unsigned)std::min<size_t>((n + 255) / 256, 4096), 256, 0, s>>>(part, out, n, nparts);
+  return hipGetLastError();
+}
+
 // nseg > 1 needs part with nseg * batch * np * nu doubles.
 hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, float* sino, int batch, int nseg,
                           double* part, hipStream_t s) {

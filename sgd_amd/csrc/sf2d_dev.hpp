@@ -53,8 +53,10 @@ __device__ __forceinline__ float sf_weight(float dx, float wx, float wy) {
 
 // A: bin-driven gather (same strip walk as fwd_bin_kernel, corner rule h = 1/2).
 // One wave: bins [64 chunk, 64 chunk + 64) of view v, wave = v * nchunk + chunk.
+// nseg > 1: lines [seg L, seg L + L) only, FP64 partial sum to part[v nu + u].
 __device__ __forceinline__ void sf_fwd_wave(const Proj2DArgs& g, const float* __restrict__ img,
-                                            float* __restrict__ sino, int wave) {
+                                            float* __restrict__ sino, int wave, int seg = 0, int nseg = 1,
+                                            double* __restrict__ part = nullptr) {
   const int lane = threadIdx.x & 63;
   const int v = wave / g.nchunk;
   if (v >= g.np) return;
@@ -94,6 +96,11 @@ __device__ __forceinline__ void sf_fwd_wave(const Proj2DArgs& g, const float* __
     m_lo = (int)fmax(floor(lo) - 1.0, 0.0);
     m_hi = (int)fmin(ceil(hi) + 1.0, (double)(nmaj - 1));
   }
+  if (nseg > 1) {
+    const int len = (nmaj + nseg - 1) / nseg;
+    m_lo = max(m_lo, seg * len);
+    m_hi = min(m_hi, seg * len + len - 1);
+  }
   double acc = 0.0;
   for (int m = m_lo; m <= m_hi; ++m) {
     const double r1 = -ra - ma * m - ac0;
@@ -131,19 +138,23 @@ __device__ __forceinline__ void sf_fwd_wave(const Proj2DArgs& g, const float* __
     }
     acc += (double)accl;
   }
-  sino[(size_t)v * g.nu + u] = (float)acc;
+  if (part) part[(size_t)v * g.nu + u] = acc;
+  else sino[(size_t)v * g.nu + u] = (float)acc;
 }
 
 
-// A^T: pixel-driven gather, one 16x16 tile per 256-thread block.
+// A^T: pixel-driven gather, one 16x16 tile per 256-thread block, views
+// [v_begin, v_end); with `part` the FP64 sums go to part[iy * nx + ix].
 __device__ __forceinline__ void sf_bwd_tile(const Proj2DArgs& g, const float* __restrict__ sino,
-                                            float* __restrict__ img, int tile_x, int tile_y) {
+                                            float* __restrict__ img, int tile_x, int tile_y, int v_begin = 0,
+                                            int v_end = -1, double* __restrict__ part = nullptr) {
   const int ix = tile_x * kTile + (threadIdx.x & (kTile - 1));
   const int iy = tile_y * kTile + threadIdx.x / kTile;
   if (ix >= g.nx || iy >= g.ny) return;
+  if (v_end < 0) v_end = g.np;
   const float fdx = (float)g.dx;
   double acc = 0.0;
-  for (int v = 0; v < g.np; ++v) {
+  for (int v = v_begin; v < v_end; ++v) {
     const ViewGeo& vg = g.views[v];
     const double wx = ix * g.dx + vg.w0x, wy = iy * g.dy + vg.w0y;
     const double n0 = vg.e0x * wy - vg.e0y * wx;
@@ -174,7 +185,8 @@ __device__ __forceinline__ void sf_bwd_tile(const Proj2DArgs& g, const float* __
     }
     acc += (double)(accv * w);
   }
-  img[(size_t)iy * g.nx + ix] = (float)acc;
+  if (part) part[(size_t)iy * g.nx + ix] = acc;
+  else img[(size_t)iy * g.nx + ix] = (float)acc;
 }
 
 

@@ -1,5 +1,6 @@
 """Timings of the workloads beside the headline pair (one JSON line each).
 
+    python -m sgd_amd.measure fan     # every 2-D basis at its workload
     python -m sgd_amd.measure batch   # 256 uneven fan slices: per-slice launches,
                                       # persistent launch, autotuned, graph replay
     python -m sgd_amd.measure cone    # cone-beam B1 / SF-TR / SF-TT, 256^3 volume,
@@ -69,6 +70,40 @@ def batch(reps=3, tune_cache=None):
     return rec
 
 
+def fan(reps=10):
+    """Every 2-D basis at its workload: B0 at config 1 (256^2, 1 mm, 360 views,
+    512 bins of 1 mm), B0 / B1 / SF at config 2 (512^2, 0.6 mm, 720 views, 768
+    bins of 0.8 mm), H0 / Hl on a 296 x 256 hexagonal lattice (dt 0.5 mm,
+    1024 views, 1024 bins of 0.6 mm) for one image and a batch of 64; lso 500,
+    lsd 1000 mm; uniform random inputs."""
+    from .torch_ops import Projector
+    recs = []
+    cases = [("B0", 256, 256, 360, 512, 1.0, 1.0, 1), ("B0", 512, 512, 720, 768, 0.6, 0.8, 1),
+             ("B1", 512, 512, 720, 768, 0.6, 0.8, 1), ("SF", 512, 512, 720, 768, 0.6, 0.8, 1),
+             ("H0", 296, 256, 1024, 1024, 0.5, 0.6, 1), ("Hl", 296, 256, 1024, 1024, 0.5, 0.6, 1),
+             ("H0", 296, 256, 1024, 1024, 0.5, 0.6, 64), ("Hl", 296, 256, 1024, 1024, 0.5, 0.6, 64)]
+    for basis, n1, n2, nv, nu, d, du, batch in cases:
+        ang = 2 * np.pi * np.arange(nv) / nv
+        if basis in ("H0", "Hl"):
+            P = Projector.hexagonal(basis, n1, n2, ang, nu, d, du, 1000.0, 500.0)
+            shape = "%dx%d hexagonal lattice (dt %.1f mm)" % (n1, n2, d)
+        else:
+            P = Projector(basis, n2, n1, ang, nu, d, d, du, 1000.0, 500.0)
+            shape = "%dx%d (%.1f mm)" % (n2, n1, d)
+        x = torch.rand((batch,) + tuple(P.img_shape), device=P.device)
+        y = torch.rand((batch,) + tuple(P.sino_shape), device=P.device)
+        yo, xo = torch.empty_like(y), torch.empty_like(x)
+        a_ms = _time(lambda: P.forward(x, out=yo), reps)
+        at_ms = _time(lambda: P.backward(y, out=xo), reps)
+        evals = P.eval_count()
+        recs.append(dict(workload="%s fan beam, %s, %d views, %d bins (%.1f mm), batch %d" % (basis, shape, nv, nu, du,
+                                                                                           batch),
+                         basis=basis, batch=batch, A_ms=a_ms, AT_ms=at_ms, evals_per_image=evals,
+                         A_evals_per_s=evals * batch / (a_ms * 1e-3)))
+        del P
+    return recs
+
+
 def cone(reps=3):
     from .torch_ops import Projector
     ang = 2 * np.pi * np.arange(360) / 360
@@ -104,13 +139,15 @@ def unet(reps=5, batch_size=8, size=512):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("what", choices=["batch", "cone", "unet"])
+    ap.add_argument("what", choices=["fan", "batch", "cone", "unet"])
     ap.add_argument("--tune-cache", default=None)
     a = ap.parse_args(argv)
     if a.what == "batch":
         recs = [batch(tune_cache=a.tune_cache)]
     elif a.what == "cone":
         recs = cone()
+    elif a.what == "fan":
+        recs = fan()
     else:
         recs = [unet()]
     for r in recs:

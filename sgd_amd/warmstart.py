@@ -19,12 +19,12 @@ Tolerance.  The Poisson noise of this workload is 0.7 % of ||b|| (relative
 ||b - b_clean|| / ||b|| of 0.0068..0.009 on seeded phantoms), and with 90 x
 768 rays against 512^2 unknowns no iterate of these methods comes near 1e-3:
 unregularised SIRT without the positivity constraint is at 2.3e-3 after 3000
-iterations, with positivity at 3.6e-3, and ASD-POCS stalls at 4e-3..8e-3
-depending on its TV weight.  The evaluation therefore stops each image at a
-multiple tau of its own noise level (the discrepancy principle; the level is
-estimated from the sinogram by the Poisson model, noise_level), tau = 2 and
-1.5, and also reports the fixed tolerance 1e-2, which a few images cannot
-reach within 3000 iterations; --tol sets others.
+iterations, with positivity at 3.6e-3, and ASD-POCS with the controls below
+at 5e-3..8e-3 (floor_study, `--phase floor`).  The evaluation therefore stops
+each image at a multiple tau of its own noise level (the discrepancy
+principle; the level is estimated from the sinogram by the Poisson model,
+noise_level), tau = 2 and 1.5, and also reports the fixed tolerance 1e-2,
+which a few images cannot reach within 3000 iterations; --tol sets others.
 
 `python -m sgd_amd.warmstart --help` runs the experiment (or one phase of it)
 and writes a JSON report.  No reference counterpart (the reference's
