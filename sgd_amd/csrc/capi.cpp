@@ -1147,7 +1147,7 @@ long long sgdct_nn_head_workspace(long long M, int C) {
 
 int sgdct_nn_head_back(const float* y, const float* target, const float* w, const void* d0, float* dy, float* gd0,
                        float* gw, float* workspace, long long workspace_floats, long long M, int C, void* stream) {
-  if (!y || !target || !w || !d0 || !dy || !gd0 || !gw || !workspace || M <= 0 || C <= 0 || C > 1023)
+  if (!y || !target || !w || !d0 || !dy || !gd0 || !gw || !workspace || M <= 0 || C <= 0 || C > 255)
     return fail(SGDCT_EINVAL, "bad head_back arguments");
   if (workspace_floats < sgdct_nn_head_workspace(M, C)) return fail(SGDCT_EINVAL, "head workspace too small");
   return nn_ret(nn_head_back(y, target, w, d0, dy, gd0, gw, workspace, M, C, (hipStream_t)stream), "head_back");

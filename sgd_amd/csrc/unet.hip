@@ -135,25 +135,172 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient.  Block: 4 waves; output tile 64 (co) x 64 (k); each wave
-// one 16-row strip of co x all 64 k (4 MFMA tiles).  The block walks pixel
-// chunks of 32 [m0, m0 + chunk) and accumulates; result -> partial[blockIdx.z].
+// LDS-tiled implicit GEMM for the same convolution (cout a multiple of 32).
+// Block tile BM pixels x BN output channels, K in steps of 32 (k = tap-major,
+// each 8-element group one tap and 8 channels since cin % 8 == 0).  Every
+// K step the block stages A (BM x 32) and B (BN x 32) in LDS with 80-byte
+// rows (16 bytes of padding: the 16-byte fragment reads of 8 consecutive rows
+// then fall in distinct banks), double-buffered: the next step's global loads
+// are in flight while the waves run their MFMAs on the current one, and one
+// barrier per step separates the two buffers' uses.  Waves WM x WN, each
+// (BM / WM) x (BN / WN) of MFMA 16x16x32 tiles.
+// ---------------------------------------------------------------------------
+constexpr int kLdsRow = 40;  // bf16 per LDS row: 32 + 8 padding
+
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256) void conv3x3_lds_kernel(ConvArgs a) {
+  static_assert(WM * WN == 4, "four waves per block");
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;  // MFMA tiles per wave
+  constexpr int AC = BM * 4 / 256;                     // A chunks (16 B) per thread
+  constexpr int BC = (BN * 4 + 255) / 256;             // B chunks per thread (some idle if BN = 32)
+  __shared__ __attribute__((aligned(16))) bf16 As[2][BM * kLdsRow];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN * kLdsRow];
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = wv % WM, wn = wv / WM;
+  const long long M = (long long)a.B * a.H * a.W;
+  const long long m_base = (long long)blockIdx.x * BM;
+  const int n_base = blockIdx.y * BN;
+  const int K = 9 * a.cin, nk = (K + 31) / 32;
+  const int r16 = lane & 15, g = lane >> 4;
+
+  // this thread's A chunks: row = q / 4, k group = q % 4, q = tid + 256 i
+  int ay[AC], ax[AC], ab[AC];
+  bool av_ok[AC];
+#pragma unroll
+  for (int i = 0; i < AC; ++i) {
+    const int q = tid + 256 * i;
+    const long long m = m_base + q / 4;
+    av_ok[i] = m < M;
+    const long long mm = av_ok[i] ? m : 0;
+    ax[i] = (int)(mm % a.W);
+    const long long t = mm / a.W;
+    ay[i] = (int)(t % a.H);
+    ab[i] = (int)(t / a.H);
+  }
+
+  bf16x8 ra[AC], rb[BC];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < AC; ++i) {
+      const int q = tid + 256 * i;
+      const int k = k0 + 8 * (q & 3);
+      bf16x8 v = zero8();
+      if (av_ok[i] && k < K) {
+        const int t = k / a.cin, c = k - t * a.cin;
+        const int yy = ay[i] + t / 3 - 1, xx = ax[i] + t % 3 - 1;
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+          v = *reinterpret_cast<const bf16x8*>(a.in + (((size_t)ab[i] * a.H + yy) * a.W + xx) * a.cin + c);
+      }
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < BC; ++i) {
+      const int q = tid + 256 * i;
+      const int n = q / 4, k = k0 + 8 * (q & 3);
+      bf16x8 v = zero8();
+      if (q < BN * 4 && n_base + n < a.cout && k < K)
+        v = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(n_base + n) * K + k);
+      rb[i] = v;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < AC; ++i) {
+      const int q = tid + 256 * i;
+      *reinterpret_cast<bf16x8*>(&As[buf][(q / 4) * kLdsRow + 8 * (q & 3)]) = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < BC; ++i) {
+      const int q = tid + 256 * i;
+      if (q < BN * 4) *reinterpret_cast<bf16x8*>(&Bs[buf][(q / 4) * kLdsRow + 8 * (q & 3)]) = rb[i];
+    }
+  };
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * 32);
+    bf16x8 fa[TM], fb[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+      fa[i] = *reinterpret_cast<const bf16x8*>(&As[buf][(wm * (BM / WM) + 16 * i + r16) * kLdsRow + 8 * g]);
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+      fb[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][(wn * (BN / WN) + 16 * j + r16) * kLdsRow + 8 * g]);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+
+  // D layout: lane holds rows 4 g + r (r = 0..3) of column r16
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int co = n_base + wn * (BN / WN) + 16 * j + r16;
+    if (co >= a.cout) continue;
+    const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long m = m_base + wm * (BM / WM) + 16 * i + 4 * g + r;
+        if (m >= M) continue;
+        float v = acc[i][j][r] + bias;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.out_f32) static_cast<float*>(a.out)[m * a.cout + co] = v;
+        else static_cast<bf16*>(a.out)[m * a.cout + co] = (bf16)v;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient dW[co][k] = sum_m dY[m][co] X[m + tap(k)][c(k)]: a GEMM
+// whose reduction runs over pixels m.  Block tile CO_T (co) x KT (k) with
+// KT = 256 / CO_T * 16 ... 4 waves, each a 16-row co strip x 64 k (4 MFMA
+// tiles).  The block walks its pixel range in chunks of 64.  Both operands
+// need 8 consecutive pixels of one channel per lane, so the chunk's dY and X
+// tiles are stored transposed in LDS ([channel][pixel]) and each MFMA
+// fragment is one 16-byte read.  Rows are 64 pixels (128 B) with the 16-byte
+// pixel groups XOR-swizzled by (row & 7) ^ ((row >> 3) & 7): the fragment
+// reads of 8 consecutive rows and the transposing 2-byte writes of 8 channel
+// groups both fall in distinct banks.  Result -> partial[blockIdx.z]
+// (pixel-range splits, added in a fixed order by reduce_partials_kernel).
 // ---------------------------------------------------------------------------
 struct WgradArgs {
   const bf16* x;    // [B][H][W][cin]
   const bf16* dy;   // [B][H][W][cout]
   float* partial;   // [nsplit][cout][9 cin]
   int B, H, W, cin, cout;
-  long long pix_per_split;
+  long long pix_per_split;  // multiple of 64
 };
 
+__device__ __forceinline__ int wg_swz(int row, int group) { return group ^ ((row & 7) ^ ((row >> 3) & 7)); }
+
+template <int CO_T>
 __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
-  __shared__ bf16 s_dy[32][64 + 8];   // [m][co]
-  __shared__ bf16 s_x[32][64 + 8];    // [m][k]
+  static_assert(CO_T == 64 || CO_T == 32, "co tile 64 or 32");
+  constexpr int WK = 64 / CO_T * 1;       // waves along k: 1 (CO_T 64) or 2 (CO_T 32)
+  constexpr int KT = 64 * WK;             // k tile
+  constexpr int DC = CO_T * 64 / 8 / 256; // dY chunks (8 channels of one pixel) per thread: 2 or 1
+  constexpr int XC = KT * 64 / 8 / 256;   // X chunks per thread: 2 or 4
+  __shared__ __attribute__((aligned(16))) bf16 s_dy[CO_T * 64];  // [co][m], swizzled groups
+  __shared__ __attribute__((aligned(16))) bf16 s_x[KT * 64];     // [k][m], swizzled groups
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r16 = lane & 15, g = lane >> 4;
+  const int wco = wv % (4 / WK), wk = wv / (4 / WK);  // this wave's co strip and k half
   const int K = 9 * a.cin;
-  const int co0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int co0 = blockIdx.x * CO_T, k0 = blockIdx.y * KT;
   const long long M = (long long)a.B * a.H * a.W;
   const long long m_begin = (long long)blockIdx.z * a.pix_per_split;
   const long long m_end = m_begin + a.pix_per_split < M ? m_begin + a.pix_per_split : M;
@@ -162,44 +309,54 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  // staging roles: thread -> (pixel row sm, 8-wide column group sc)
-  const int sm = threadIdx.x >> 3, sc = (threadIdx.x & 7) * 8;
-  for (long long m0 = m_begin; m0 < m_end; m0 += 32) {
-    {
-      const long long m = m0 + sm;
-      const bool mv = m < m_end;
-      // dY tile: channels co0 + sc .. +7
-      bf16x8 v = zero8();
-      if (mv && co0 + sc < a.cout) v = *reinterpret_cast<const bf16x8*>(a.dy + m * a.cout + co0 + sc);
+  for (long long m0 = m_begin; m0 < m_end; m0 += 64) {
+    // stage: chunk q -> pixel pm = q % 64, channel group cg = (q / 64) * 8
 #pragma unroll
-      for (int e = 0; e < 8; ++e) s_dy[sm][sc + e] = v[e];
-      // X tile: k0 + sc .. +7 -> one tap, 8 channels, at pixel m shifted by the tap
-      bf16x8 xvv = zero8();
-      const int k = k0 + sc;
-      if (mv && k < K) {
+    for (int i = 0; i < DC; ++i) {
+      const int q = threadIdx.x + 256 * i, pm = q & 63, cg = (q >> 6) * 8;
+      const long long m = m0 + pm;
+      bf16x8 v = zero8();
+      if (m < m_end && co0 + cg < a.cout) v = *reinterpret_cast<const bf16x8*>(a.dy + m * a.cout + co0 + cg);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int row = cg + e;
+        s_dy[row * 64 + wg_swz(row, pm >> 3) * 8 + (pm & 7)] = v[e];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < XC; ++i) {
+      const int q = threadIdx.x + 256 * i, pm = q & 63, cg = (q >> 6) * 8;
+      const long long m = m0 + pm;
+      const int k = k0 + cg;
+      bf16x8 v = zero8();
+      if (m < m_end && k < K) {
         const int t = k / a.cin, c = k - t * a.cin;
         const int x = (int)(m % a.W);
-        const long long q = m / a.W;
-        const int y = (int)(q % a.H);
-        const int b = (int)(q / a.H);
+        const long long qq = m / a.W;
+        const int y = (int)(qq % a.H);
+        const int b = (int)(qq / a.H);
         const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
         if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-          xvv = *reinterpret_cast<const bf16x8*>(a.x + (((size_t)b * a.H + yy) * a.W + xx) * a.cin + c);
+          v = *reinterpret_cast<const bf16x8*>(a.x + (((size_t)b * a.H + yy) * a.W + xx) * a.cin + c);
       }
 #pragma unroll
-      for (int e = 0; e < 8; ++e) s_x[sm][sc + e] = xvv[e];
+      for (int e = 0; e < 8; ++e) {
+        const int row = cg + e;
+        s_x[row * 64 + wg_swz(row, pm >> 3) * 8 + (pm & 7)] = v[e];
+      }
     }
     __syncthreads();
-    // A[co][m] = s_dy[m][co]: lane row co = 16 wv + r16, m = 8 g .. 8 g + 7
-    bf16x8 av;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) av[e] = s_dy[8 * g + e][16 * wv + r16];
+    for (int h = 0; h < 2; ++h) {  // two 32-pixel MFMA steps
+      const int grp = 4 * h + g;     // this lane's 8-pixel group
+      const int ra = 16 * wco + r16;
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&s_dy[ra * 64 + wg_swz(ra, grp) * 8]);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      bf16x8 bv;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) bv[e] = s_x[8 * g + e][16 * j + r16];
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j) {
+        const int rb = 64 * wk + 16 * j + r16;
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&s_x[rb * 64 + wg_swz(rb, grp) * 8]);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
@@ -208,7 +365,7 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
   for (int j = 0; j < 4; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int co = co0 + 16 * wv + 4 * g + r, k = k0 + 16 * j + r16;
+      const int co = co0 + 16 * wco + 4 * g + r, k = k0 + 64 * wk + 16 * j + r16;
       if (co < a.cout && k < K) out[(size_t)co * K + k] = acc[j][r];
     }
 }
@@ -218,22 +375,52 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial, float*
                                        int nsplit) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float s = 0.0f;
-  for (int k = 0; k < nsplit; ++k) s += partial[(size_t)k * n + i];
-  out[i] = s;
+  float p4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int k = 0;
+  for (; k + 3 < nsplit; k += 4) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p4[j] += partial[(size_t)(k + j) * n + i];
+  }
+  for (int j = 0; k < nsplit; ++k, ++j) p4[j & 3] += partial[(size_t)k * n + i];
+  out[i] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
 }
 
-// Column sums of an [M][C] bf16 or FP32 matrix into partial[split][C].
+// Column sums of an [M][C] bf16 or FP32 matrix into partial[split][C]: block
+// `split` reduces rows [split rpb, split rpb + rpb).  Up to 256 columns per
+// pass; the 256 threads are (256 / cw) row lanes x cw columns, so a row's
+// columns are read by consecutive threads, and the row lanes are added in a
+// fixed order through LDS.  blockDim.x = 256.
 template <typename T>
-__global__ void colsum_kernel(const T* __restrict__ a, float* __restrict__ partial, long long M, int C,
-                              long long rows_per_split) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
-  const long long r0 = (long long)blockIdx.y * rows_per_split;
+__global__ __launch_bounds__(256) void colsum_kernel(const T* __restrict__ a, float* __restrict__ partial, long long M,
+                                                     int C, long long rows_per_split) {
+  __shared__ float red[256];
+  const long long r0 = (long long)blockIdx.x * rows_per_split;
   const long long r1 = r0 + rows_per_split < M ? r0 + rows_per_split : M;
-  float s = 0.0f;
-  for (long long r = r0; r < r1; ++r) s += (float)a[r * C + c];
-  partial[(size_t)blockIdx.y * C + c] = s;
+  for (int c0 = 0; c0 < C; c0 += 256) {
+    const int cw = C - c0 < 256 ? C - c0 : 256, nrl = 256 / cw;
+    const int rl = threadIdx.x / cw, c = c0 + (int)threadIdx.x % cw;
+    // eight independent partial sums keep eight loads in flight
+    float p8[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (rl < nrl) {
+      long long r = r0 + rl;
+      for (; r + 7LL * nrl < r1; r += 8LL * nrl) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p8[j] += (float)a[(r + (long long)j * nrl) * C + c];
+      }
+      for (int j = 0; r < r1; r += nrl, ++j) p8[j & 7] += (float)a[r * C + c];
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += p8[j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < cw) {
+      float t = 0.0f;
+      for (int k = 0; k < nrl; ++k) t += red[k * cw + threadIdx.x];
+      partial[(size_t)blockIdx.x * C + c0 + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -345,14 +532,39 @@ __global__ void head_back_kernel(const float* __restrict__ y, const float* __res
 }
 
 // head weight gradient partials: part[s][c] = sum_m dy[m] d0[m][c], part[s][C] = sum_m dy[m]
-__global__ void head_wgrad_kernel(const float* __restrict__ dy, const bf16* __restrict__ d0, float* __restrict__ part,
-                                  long long M, int C, long long rows_per_split) {
-  const int c = threadIdx.x;  // blockDim.x = C + 1
+// partial[split][c] = sum over the split's rows of dy[r] * d0[r][c] (c < C)
+// or dy[r] (c = C, the bias); C + 1 <= 256 columns, row lanes as in
+// colsum_kernel.  blockDim.x = 256.
+__global__ __launch_bounds__(256) void head_wgrad_kernel(const float* __restrict__ dy, const bf16* __restrict__ d0,
+                                                         float* __restrict__ part, long long M, int C,
+                                                         long long rows_per_split) {
+  __shared__ float red[256];
+  const int cw = C + 1, nrl = 256 / cw;
+  const int rl = threadIdx.x / cw, c = (int)threadIdx.x % cw;
   const long long r0 = (long long)blockIdx.x * rows_per_split;
   const long long r1 = r0 + rows_per_split < M ? r0 + rows_per_split : M;
+  float p8[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (rl < nrl) {
+    long long r = r0 + rl;
+    for (; r + 7LL * nrl < r1; r += 8LL * nrl) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const long long rr = r + (long long)j * nrl;
+        p8[j] += dy[rr] * (c < C ? (float)d0[rr * C + c] : 1.0f);
+      }
+    }
+    for (int j = 0; r < r1; r += nrl, ++j) p8[j & 7] += dy[r] * (c < C ? (float)d0[r * C + c] : 1.0f);
+  }
   float s = 0.0f;
-  for (long long r = r0; r < r1; ++r) s += dy[r] * (c < C ? (float)d0[r * C + c] : 1.0f);
-  part[(size_t)blockIdx.x * (C + 1) + c] = s;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s += p8[j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if ((int)threadIdx.x < cw) {
+    float t = 0.0f;
+    for (int k = 0; k < nrl; ++k) t += red[k * cw + threadIdx.x];
+    part[(size_t)blockIdx.x * cw + threadIdx.x] = t;
+  }
 }
 
 // out = bf16(g * (act > 0))
@@ -410,8 +622,13 @@ hipError_t nn_conv3x3(const void* in, const void* w, const float* bias, void* ou
                       int cout, int relu, int out_f32, hipStream_t s) {
   ConvArgs a{static_cast<const bf16*>(in), static_cast<const bf16*>(w), bias, out, B, H, W, cin, cout, relu, out_f32};
   const long long M = (long long)B * H * W;
-  if (cout % 64 == 0) {
-    conv3x3_kernel<2, 2><<<dim3(blocks_for(M, 64), cout / 64), 256, 0, s>>>(a);
+  if (cin % 8 != 0) return hipErrorInvalidValue;
+  if (cout % 128 == 0) {
+    conv3x3_lds_kernel<128, 128, 2, 2><<<dim3(blocks_for(M, 128), cout / 128), 256, 0, s>>>(a);
+  } else if (cout % 64 == 0) {
+    conv3x3_lds_kernel<128, 64, 2, 2><<<dim3(blocks_for(M, 128), cout / 64), 256, 0, s>>>(a);
+  } else if (cout % 32 == 0) {
+    conv3x3_lds_kernel<256, 32, 4, 1><<<dim3(blocks_for(M, 256), cout / 32), 256, 0, s>>>(a);
   } else {
     conv3x3_kernel<4, 1><<<dim3(blocks_for(M, 128), (cout + 31) / 32), 256, 0, s>>>(a);
   }
@@ -428,10 +645,11 @@ hipError_t nn_wgrad(const void* x, const void* dy, float* dw, float* partial, in
                     hipStream_t s) {
   const long long M = (long long)B * H * W;
   const long long ns = nn_wgrad_splits(M);
-  const long long per = ((M + ns - 1) / ns + 31) / 32 * 32;
+  const long long per = ((M + ns - 1) / ns + 63) / 64 * 64;
   const int K = 9 * cin;
   WgradArgs a{static_cast<const bf16*>(x), static_cast<const bf16*>(dy), partial, B, H, W, cin, cout, per};
-  wgrad_kernel<<<dim3((cout + 63) / 64, (K + 63) / 64, (unsigned)ns), 256, 0, s>>>(a);
+  if (cout <= 32) wgrad_kernel<32><<<dim3(1, (K + 127) / 128, (unsigned)ns), 256, 0, s>>>(a);
+  else wgrad_kernel<64><<<dim3((cout + 63) / 64, (K + 63) / 64, (unsigned)ns), 256, 0, s>>>(a);
   const long long n = (long long)cout * K;
   reduce_partials_kernel<<<blocks_for(n), 256, 0, s>>>(partial, dw, n, (int)ns);
   return hipGetLastError();
@@ -439,8 +657,7 @@ hipError_t nn_wgrad(const void* x, const void* dy, float* dw, float* partial, in
 
 hipError_t nn_colsum_bf16(const void* a, float* out, float* partial, long long M, int C, hipStream_t s) {
   const long long ns = (M + 4095) / 4096;
-  colsum_kernel<bf16><<<dim3((C + 63) / 64, (unsigned)ns), 64, 0, s>>>(static_cast<const bf16*>(a), partial, M, C,
-                                                                      4096);
+  colsum_kernel<bf16><<<(unsigned)ns, 256, 0, s>>>(static_cast<const bf16*>(a), partial, M, C, 4096);
   reduce_partials_kernel<<<blocks_for(C), 256, 0, s>>>(partial, out, C, (int)ns);
   return hipGetLastError();
 }
@@ -485,7 +702,7 @@ hipError_t nn_head_back(const float* y, const float* target, const float* w, con
                         float* gw, float* partial, long long M, int C, hipStream_t s) {
   head_back_kernel<<<blocks_for(M), 256, 0, s>>>(y, target, w, dy, gd0, M, C, 2.0f / (float)M);
   const long long ns = (M + 4095) / 4096;
-  head_wgrad_kernel<<<(unsigned)ns, C + 1, 0, s>>>(dy, static_cast<const bf16*>(d0), partial, M, C, 4096);
+  head_wgrad_kernel<<<(unsigned)ns, 256, 0, s>>>(dy, static_cast<const bf16*>(d0), partial, M, C, 4096);
   reduce_partials_kernel<<<blocks_for(C + 1), 256, 0, s>>>(partial, gw, C + 1, (int)ns);
   return hipGetLastError();
 }
