@@ -133,6 +133,27 @@ def test_config1_full(basis, cuda):
     assert rel(xa, xe) <= TOL_EXACT
 
 
+@pytest.mark.parametrize("basis", ["B0", "B1", "SF"])
+def test_split_launches_agree(basis, cuda):
+    """A with 1..16 line segments and A^T with 1..16 view groups (FP64
+    partial sums added in a fixed order) give the unsplit result to one FP32
+    rounding, and each split is deterministic."""
+    import torch
+    P, ge, gr = make(basis, "config1", cuda)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(P.img_shape, device="cuda", generator=g)
+    y = torch.rand(P.sino_shape, device="cuda", generator=g)
+    P.set_split(1, 1)
+    y1, x1 = P.forward(x), P.backward(y)
+    for fs, bs in ((2, 3), (7, 5), (16, 16), (0, 0)):
+        P.set_split(fs, bs)
+        ys, xs = P.forward(x), P.backward(y)
+        torch.cuda.synchronize()
+        assert torch.allclose(ys, y1, rtol=2.4e-7, atol=1e-30), (fs, bs)
+        assert torch.allclose(xs, x1, rtol=2.4e-7, atol=1e-30), (fs, bs)
+        assert torch.equal(P.forward(x), ys) and torch.equal(P.backward(y), xs), (fs, bs)
+
+
 def test_batch_and_determinism(cuda):
     import torch
     P, ge, gr = make("B1", "tiny", cuda)
