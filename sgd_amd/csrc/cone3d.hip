@@ -47,7 +47,11 @@
 namespace sgdct {
 
 constexpr int kMaxNz = 512;        // voxels per column staged in LDS
-constexpr int kMaxRowChunks = 8;   // detector rows per lane: nv <= 512
+// The in-plane kernels give each wave RCW chunks of 64 detector rows (a grid
+// dimension runs over the chunk groups): with RCW < nv / 64 more waves are in
+// flight on the row profiles, at the price of recomputing the in-plane weight
+// once per group.  RCW = kAllRows keeps every row in one wave (nv <= 512).
+constexpr int kAllRows = 8;
 constexpr float kGammaEps = 1e-7f; // A_B1_3D.cu:300 skips gamma <= eps
 
 enum ConeModel : int { CONE_B1 = 0, CONE_SF_TR = 1, CONE_SF_TT = 2 };
@@ -213,11 +217,12 @@ __device__ __forceinline__ float sf_cell_weight(const ViewGeo& vg, const Proj2DA
   return width > 0.0f ? gm * 2.0f / width : 0.0f;
 }
 
-template <int M>
+template <int M, int RCW>
 __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int nview, const float* __restrict__ Y,
                                                        float* __restrict__ proj) {
   const Proj2DArgs& g = a.g;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kc0 = blockIdx.y * RCW;  // this wave's detector-row chunks
   const int pl = blockIdx.x % nview;
   const int u = (blockIdx.x / nview) * 4 + wv;
   if (u >= g.nu) return;
@@ -268,19 +273,19 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
   const double an = (double)rec.ka * sa, am = (double)rec.ka * ma, a0 = (double)rec.ka * ac0;
   const double bn = (double)rec.kb * sb, bm = (double)rec.kb * mb, b0 = (double)rec.kb * bc0;
 
-  double acc[kMaxRowChunks];
+  double acc[RCW];
 #pragma unroll
-  for (int k = 0; k < kMaxRowChunks; ++k) acc[k] = 0.0;
-  const float* ybase = Y + (size_t)pl * ncol * a.nv + lane;
+  for (int k = 0; k < RCW; ++k) acc[k] = 0.0;
+  const float* ybase = Y + (size_t)pl * ncol * a.nv + kc0 * 64 + lane;
   for (int m = m_lo; m <= m_hi; ++m) {
     const int n0 = (int)fmax(ceil(fma((double)m, lo1, lo0)), 0.0);
     const int n1 = (int)fmin(floor(fma((double)m, hi1, hi0)), (double)(nmin - 1));
     if (n0 > n1) continue;
     float ta = (float)fma((double)n0, an, fma((double)m, am, a0));
     float tb = (float)fma((double)n0, bn, fma((double)m, bm, b0));
-    float accl[kMaxRowChunks];
+    float accl[RCW];
 #pragma unroll
-    for (int k = 0; k < kMaxRowChunks; ++k) accl[k] = 0.0f;
+    for (int k = 0; k < RCW; ++k) accl[k] = 0.0f;
     for (int n = n0; n <= n1; ++n) {
       float w;
       if (M == CONE_B1) {
@@ -292,19 +297,19 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
       }
       const float* yc = ybase + (size_t)(m * smaj + n * smin) * a.nv;
 #pragma unroll
-      for (int k = 0; k < kMaxRowChunks; ++k)
-        if (k < a.nrc && k * 64 + lane < a.nv) accl[k] = __builtin_fmaf(w, yc[k * 64], accl[k]);
+      for (int k = 0; k < RCW; ++k)
+        if ((kc0 + k) * 64 + lane < a.nv) accl[k] = __builtin_fmaf(w, yc[k * 64], accl[k]);
       ta += dta;
       tb += dtb;
     }
 #pragma unroll
-    for (int k = 0; k < kMaxRowChunks; ++k) acc[k] += (double)accl[k];
+    for (int k = 0; k < RCW; ++k) acc[k] += (double)accl[k];
   }
   float* out = proj + (size_t)v * a.nv * g.nu + u;
 #pragma unroll
-  for (int k = 0; k < kMaxRowChunks; ++k) {
-    const int j = k * 64 + lane;
-    if (k < a.nrc && j < a.nv) out[(size_t)j * g.nu] = (float)acc[k];
+  for (int k = 0; k < RCW; ++k) {
+    const int j = (kc0 + k) * 64 + lane;
+    if (j < a.nv) out[(size_t)j * g.nu] = (float)acc[k];
   }
 }
 
@@ -312,13 +317,14 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
 // Yb[pl][c][j] = sum_u conv(u, c) projT[pl][u][j].  Wave per (column, view),
 // the column's clamped corner range (A_B1_3D.cu:117-151) wave-uniform.
 // ---------------------------------------------------------------------------
-template <int M>
+template <int M, int RCW>
 __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, const float* __restrict__ projT,
                                                        float* __restrict__ Yb) {
   const Proj2DArgs& g = a.g;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ncol = g.nx * g.ny;
   const int c = blockIdx.x * 4 + wv, pl = blockIdx.y;
+  const int kc0 = blockIdx.z * RCW;  // this wave's detector-row chunks
   if (c >= ncol) return;
   const int v = p0 + pl;
   const int ix = c % g.nx, iy = c / g.nx;
@@ -334,9 +340,9 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
   const float last = (float)(g.nu - 1);
   const int ulo = (int)fminf(fmaxf(floorf(kc - hw), 0.0f), last);
   const int uhi = (int)fminf(fmaxf(floorf(kc + hw), 0.0f), last);
-  float acc[kMaxRowChunks];
+  float acc[RCW];
 #pragma unroll
-  for (int k = 0; k < kMaxRowChunks; ++k) acc[k] = 0.0f;
+  for (int k = 0; k < RCW; ++k) acc[k] = 0.0f;
   float na = (float)fma((double)ulo, be, n0);
   for (int u = ulo; u <= uhi; ++u) {
     const float nb = na + bef;
@@ -349,16 +355,16 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
     } else {
       w = sf_cell_weight(vg, g, u, ix, iy);
     }
-    const float* row = projT + ((size_t)pl * g.nu + u) * a.nv + lane;
+    const float* row = projT + ((size_t)pl * g.nu + u) * a.nv + kc0 * 64 + lane;
 #pragma unroll
-    for (int k = 0; k < kMaxRowChunks; ++k)
-      if (k < a.nrc && k * 64 + lane < a.nv) acc[k] = __builtin_fmaf(w, row[k * 64], acc[k]);
+    for (int k = 0; k < RCW; ++k)
+      if ((kc0 + k) * 64 + lane < a.nv) acc[k] = __builtin_fmaf(w, row[k * 64], acc[k]);
     na = nb;
   }
-  float* out = Yb + ((size_t)pl * ncol + c) * a.nv + lane;
+  float* out = Yb + ((size_t)pl * ncol + c) * a.nv + kc0 * 64 + lane;
 #pragma unroll
-  for (int k = 0; k < kMaxRowChunks; ++k)
-    if (k < a.nrc && k * 64 + lane < a.nv) out[k * 64] = acc[k];
+  for (int k = 0; k < RCW; ++k)
+    if ((kc0 + k) * 64 + lane < a.nv) out[k * 64] = acc[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -436,14 +442,17 @@ static void cone_forward_chunk(const Cone3DArgs& a, int p0, int nview, const flo
                                hipStream_t s) {
   const int ncol = a.g.nx * a.g.ny;
   axial_fwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, xT, Y);
-  conv_fwd_kernel<M><<<dim3(((a.g.nu + 3) / 4) * nview), dim3(256), 0, s>>>(a, p0, nview, Y, proj);
+  // B1's in-plane weight is cheap: split the rows; SF's (four divisions) is not
+  constexpr int RCW = M == CONE_B1 ? 2 : kAllRows;
+  const int groups = (a.nrc + RCW - 1) / RCW;
+  conv_fwd_kernel<M, RCW><<<dim3(((a.g.nu + 3) / 4) * nview, groups), dim3(256), 0, s>>>(a, p0, nview, Y, proj);
 }
 
 template <int M>
 static void cone_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb, float* xT,
                                 int accumulate, hipStream_t s) {
   const int ncol = a.g.nx * a.g.ny;
-  conv_bwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, projT, Yb);
+  conv_bwd_kernel<M, kAllRows><<<dim3((ncol + 3) / 4, nview, 1), dim3(256), 0, s>>>(a, p0, projT, Yb);
   axial_bwd_kernel<M><<<dim3((ncol + 3) / 4), dim3(256), 0, s>>>(a, p0, nview, Yb, xT, accumulate);
 }
 
