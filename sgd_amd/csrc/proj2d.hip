@@ -7,6 +7,12 @@
 //                    pixel tiles per workgroup; per-bin constants and the
 //                    sinogram values of the bins the tile sees are staged in
 //                    LDS, kViewChunk views at a time.  Deterministic gather.
+//   sum_parts_kernel adds the FP64 partial results of split launches in a
+//                    fixed order: A may split every bin's walk into line
+//                    segments and A^T the views into groups (grid z), so that
+//                    a single image fills the GPU; each segment / group
+//                    writes its partial sums, and the split is chosen per
+//                    projector from one image's launch size.
 //   count_kernel     number of footprint evaluations under the reference's
 //                    corner rule (the value layers/A_B1.cu:50,643 returns).
 //
