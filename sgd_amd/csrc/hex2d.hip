@@ -120,8 +120,19 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
   const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
   const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
 
+  // rows the strip can reach: between the crossings of its two boundary
+  // lines with the first and last column (ic' = 0 and nc), one row of margin
+  int il_lo = 0, il_hi = g.ny - 1;
+  if (mq != 0.0 && mr != 0.0) {
+    const double last = (double)g.nx;
+    const double c1 = (-rq - cq) / mq, c2 = (-rq - cq - sq * last) / mq;
+    const double c3 = (rr - cr) / mr, c4 = (rr - cr - sr * last) / mr;
+    const double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
+    il_lo = (int)fmax(floor(lo) - 1.0, 0.0);
+    il_hi = (int)fmin(ceil(hi) + 1.0, (double)(g.ny - 1));
+  }
   double acc = 0.0;
-  for (int il = 0; il < g.ny; ++il) {
+  for (int il = il_lo; il <= il_hi; ++il) {
     const double sh = (il & 1) ? 0.5 : 0.0;
     const double r1 = -rq - mq * il - cq - sq * sh;  // need sq * ic >= r1
     const double r2 = rr - mr * il - cr - sr * sh;   // need sr * ic <  r2
