@@ -29,3 +29,12 @@ def test_binding_sets_signatures():
     L = _lib.lib()
     assert L.b1_init.restype is ctypes.c_void_p
     assert L.sgdct_last_error.restype is ctypes.c_char_p
+
+
+def test_split_config_rejects_bad_arguments_without_a_device():
+    """Argument checks run before any device work: a null handle or a split
+    outside 0..16 is SGDCT_EINVAL with a message."""
+    L = _lib.lib()
+    assert L.sgdct_split_config(None, 0, 0) == -1
+    assert "split" in _lib.last_error()
+    assert L.sgdct_split_config(None, 17, 0) == -1
