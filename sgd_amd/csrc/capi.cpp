@@ -200,6 +200,7 @@ namespace {
 int auto_fwd_split(const sgdct_projector* h);
 int auto_bwd_split(const sgdct_projector* h);
 int ensure_part(sgdct_projector* h, size_t doubles);
+void release_part(sgdct_projector* h);
 size_t part_need(const sgdct_projector* h);
 // images per launch the partial-sum workspace holds
 constexpr int kPartImages = 2;
@@ -449,6 +450,13 @@ int ensure_part(sgdct_projector* h, size_t doubles) {
   if (e != hipSuccess) return hip_fail(e, "partial-sum workspace");
   h->part_cap = doubles;
   return SGDCT_OK;
+}
+
+void release_part(sgdct_projector* h) {
+  if (h->d_part) (void)hipFree(h->d_part);
+  h->d_part = nullptr;
+  h->part_cap = 0;
+  h->fwd_auto = h->bwd_auto = 1;
 }
 
 // Workspace of one image at the current splits.
@@ -843,6 +851,7 @@ sgdct_projector* b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, doub
   }
   sgdct_projector* h = create(BASIS_B1, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, nullptr);
   if (!h) return nullptr;
+  release_part(h);  // the cone-beam kernels do not split their launches
   h->basis = BASIS_B1_3D;
   h->nz = nz;
   h->nv = nv;
@@ -873,6 +882,7 @@ sgdct_projector* sf3d_init(int nx, int ny, int nz, int np, int nu, int nv, doubl
   }
   sgdct_projector* h = create(BASIS_SF, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, nullptr, 0.0, 0.5);
   if (!h) return nullptr;
+  release_part(h);
   h->basis = BASIS_SF_3D;
   h->cone_model = trap_v ? 2 : 1;
   h->nz = nz;
