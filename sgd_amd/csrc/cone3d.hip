@@ -64,6 +64,7 @@ enum ConeModel : int { CONE_B1 = 0, CONE_SF_TR = 1, CONE_SF_TT = 2 };
 // SF: lsd^2 dx dy dz / (du dv r^2) / area, SF_projector.cu:193-198, 364).
 struct AxialTrap {
   float v0, v1, v2, v3, cf;
+  float i01, i23;  // 1 / (2 (v1 - v0)), 1 / (2 (v3 - v2)); inf on a flat side, which is never used
 };
 
 struct ColumnAxial {
@@ -110,6 +111,8 @@ __device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const Colum
     const float area = (M == CONE_SF_TR) ? (t.v3 - t.v0) : 0.5f * ((t.v3 - t.v0) + (t.v2 - t.v1));
     t.cf = a.fac / ((c.r2 + oz * oz) * area);
   }
+  t.i01 = 0.5f / (t.v1 - t.v0);
+  t.i23 = 0.5f / (t.v3 - t.v2);
   return t;
 }
 
@@ -121,16 +124,18 @@ __device__ __forceinline__ float row_off() {
 
 // Overlap of [s, s+1) with the unit-height trapezoid (A_SF.cu:48-84 /
 // A_B1_3D.cu:30-66), segment by segment.
+// The sloped sides divide by their widths through the voxel's precomputed
+// reciprocals (a side of zero width is never entered: b2 > b1 fails).
 __device__ __forceinline__ float trap_gamma(float s, const AxialTrap& t) {
   const float e = s + 1.0f;
   float b1 = fmaxf(s, t.v0), b2 = fminf(e, t.v1);
-  float g0 = (b2 > b1) ? (b2 - b1) * (b2 + b1 - 2.0f * t.v0) / (2.0f * (t.v1 - t.v0)) : 0.0f;
+  float g0 = (b2 > b1) ? (b2 - b1) * (b2 + b1 - 2.0f * t.v0) * t.i01 : 0.0f;
   b1 = fmaxf(s, t.v1);
   b2 = fminf(e, t.v2);
   float g1 = fmaxf(b2 - b1, 0.0f);
   b1 = fmaxf(s, t.v2);
   b2 = fminf(e, t.v3);
-  float g2 = (b2 > b1) ? (b2 - b1) * (2.0f * t.v3 - b1 - b2) / (2.0f * (t.v3 - t.v2)) : 0.0f;
+  float g2 = (b2 > b1) ? (b2 - b1) * (2.0f * t.v3 - b1 - b2) * t.i23 : 0.0f;
   return g0 + g1 + g2;
 }
 
@@ -143,11 +148,23 @@ __device__ __forceinline__ bool keep_gamma(float gm) {
 // Y[pl][c][j] = sum_iz xT[c][iz] * cfac * gamma_j.  Block: 4 waves, one
 // column each; grid (ceil(ncol / 4), views in the chunk).
 // ---------------------------------------------------------------------------
+// Dynamic LDS: per wave kAxialFields arrays of nz floats (v0..v3, x * cf,
+// i01, i23).
+constexpr int kAxialFields = 7;
+
 template <int M>
 __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, const float* __restrict__ xT,
                                                         float* __restrict__ Y) {
-  __shared__ float lds[4][5][kMaxNz];
+  extern __shared__ float axial_lds[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float* const L = axial_lds + (size_t)wv * kAxialFields * a.nz;  // this wave's arrays
+  float* const Lv0 = L;
+  float* const Lv1 = L + a.nz;
+  float* const Lv2 = L + 2 * a.nz;
+  float* const Lv3 = L + 3 * a.nz;
+  float* const Lxc = L + 4 * a.nz;
+  float* const Li01 = L + 5 * a.nz;
+  float* const Li23 = L + 6 * a.nz;
   const int ncol = a.g.nx * a.g.ny;
   const int c = blockIdx.x * 4 + wv, pl = blockIdx.y;
   const bool live = c < ncol;
@@ -156,11 +173,13 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
     const ColumnAxial ca = column_axial<M>(a, a.g.views[p0 + pl], ix, iy);
     for (int iz = lane; iz < a.nz; iz += 64) {
       const AxialTrap t = voxel_trap<M>(a, ca, iz);
-      lds[wv][0][iz] = t.v0;
-      lds[wv][1][iz] = t.v1;
-      lds[wv][2][iz] = t.v2;
-      lds[wv][3][iz] = t.v3;
-      lds[wv][4][iz] = xT[(size_t)c * a.nz + iz] * t.cf;
+      Lv0[iz] = t.v0;
+      Lv1[iz] = t.v1;
+      Lv2[iz] = t.v2;
+      Lv3[iz] = t.v3;
+      Lxc[iz] = xT[(size_t)c * a.nz + iz] * t.cf;
+      Li01[iz] = t.i01;
+      Li23[iz] = t.i23;
     }
   }
   __syncthreads();
@@ -172,18 +191,20 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
     int lo = 0, hi = a.nz;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      if (lds[wv][3][mid] >= fj) hi = mid;
+      if (Lv3[mid] >= fj) hi = mid;
       else lo = mid + 1;
     }
     float acc = 0.0f;
-    for (int iz = lo; iz < a.nz && lds[wv][0][iz] < fj + 1.0f; ++iz) {
+    for (int iz = lo; iz < a.nz && Lv0[iz] < fj + 1.0f; ++iz) {
       AxialTrap t;
-      t.v0 = lds[wv][0][iz];
-      t.v1 = lds[wv][1][iz];
-      t.v2 = lds[wv][2][iz];
-      t.v3 = lds[wv][3][iz];
+      t.v0 = Lv0[iz];
+      t.v1 = Lv1[iz];
+      t.v2 = Lv2[iz];
+      t.v3 = Lv3[iz];
+      t.i01 = Li01[iz];
+      t.i23 = Li23[iz];
       const float gm = trap_gamma(fj, t);
-      if (keep_gamma<M>(gm)) acc = __builtin_fmaf(lds[wv][4][iz], gm, acc);
+      if (keep_gamma<M>(gm)) acc = __builtin_fmaf(Lxc[iz], gm, acc);
     }
     yrow[j] = acc;
   }
@@ -441,7 +462,8 @@ template <int M>
 static void cone_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y, float* proj,
                                hipStream_t s) {
   const int ncol = a.g.nx * a.g.ny;
-  axial_fwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), 0, s>>>(a, p0, xT, Y);
+  const size_t lds = sizeof(float) * 4 * kAxialFields * a.nz;
+  axial_fwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), lds, s>>>(a, p0, xT, Y);
   // B1's in-plane weight is cheap: split the rows; SF's (four divisions) is not
   constexpr int RCW = M == CONE_B1 ? 2 : kAllRows;
   const int groups = (a.nrc + RCW - 1) / RCW;

@@ -118,7 +118,7 @@ def make_pairs(P, seeds, batch=64, sirt_iters=5, asd_iters=200, log=print):
     return torch.cat(ins), torch.cat(outs), torch.cat(sinos)
 
 
-def train(net, inputs, targets, epochs=3, batch=8, lr=1e-3, seed=0, log=print):
+def train(net, inputs, targets, epochs=8, batch=8, lr=1e-3, seed=0, log=print):
     n = inputs.shape[0]
     rng = np.random.default_rng(seed)
     losses = []
@@ -238,7 +238,7 @@ def main(argv=None):
     ap.add_argument("--weights", default=None, help="weights file (.npz) written by train / read by eval")
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--test", type=int, default=256)
-    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--tol", default="2s,1.5s,1e-2",
