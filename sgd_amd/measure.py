@@ -1,6 +1,7 @@
 """Timings of the workloads beside the headline pair (one JSON line each).
 
     python -m sgd_amd.measure fan     # every 2-D basis at its workload
+    python -m sgd_amd.measure asd     # ASD-POCS, 50 iterations at the 512^2 / 720-view workload
     python -m sgd_amd.measure batch   # 256 uneven fan slices: per-slice launches,
                                       # persistent launch, autotuned, graph replay
     python -m sgd_amd.measure cone    # cone-beam B1 / SF-TR / SF-TT, 256^3 volume,
@@ -104,6 +105,35 @@ def fan(reps=10):
     return recs
 
 
+def asd(niter=50):
+    """Plain ASD-POCS at config 2: B1, 512^2 phantom (0.6 mm), 720 views,
+    768 bins (0.8 mm), Poisson noise at 1e5 photons on p = 0.1 A x, 50
+    iterations with TV step ratio alpha = 0.2 (Sidky & Pan's controls); SIRT
+    beside it at the same iteration count."""
+    from .phantom import phantom, poisson_noise
+    from .solvers import SirtWeights, asd_pocs, sirt
+    from .torch_ops import Projector
+    ang = 2 * np.pi * np.arange(720) / 720
+    P = Projector("B1", 512, 512, ang, 768, 0.6, 0.6, 0.8, 1000.0, 500.0)
+    xt = torch.tensor(phantom(0, 512, 512, 0.6, 0.6), dtype=torch.float32, device=P.device)
+    y = P.forward(xt).double().cpu().numpy()
+    b = torch.tensor(poisson_noise(0.1 * y, 1e5, seed=1) / 0.1, dtype=torch.float32, device=P.device)
+    w = SirtWeights(P)
+    asd_pocs(P, b, niter=2, weights=w)  # warm-up
+    r = asd_pocs(P, b, niter=niter, alpha=0.2, weights=w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs, hist = sirt(P, b, niter, weights=w)
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    err = lambda x: float(torch.linalg.vector_norm((x - xt).double()) / torch.linalg.vector_norm(xt.double()))
+    return dict(workload="ASD-POCS, B1 fan beam, 512^2 (0.6 mm), 720 views, 768 bins (0.8 mm), Poisson 1e5, "
+                         "%d iterations, TV step ratio 0.2, 20 TV steps per iteration" % niter,
+                asd_seconds=r.seconds, asd_ms_per_iteration=1e3 * r.seconds / niter,
+                asd_final_residual=r.residuals[-1], asd_image_error=err(r.x),
+                sirt_seconds=ts, sirt_final_residual=hist[-1], sirt_image_error=err(xs))
+
+
 def cone(reps=3):
     from .torch_ops import Projector
     ang = 2 * np.pi * np.arange(360) / 360
@@ -139,7 +169,7 @@ def unet(reps=5, batch_size=8, size=512):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("what", choices=["fan", "batch", "cone", "unet"])
+    ap.add_argument("what", choices=["fan", "asd", "batch", "cone", "unet"])
     ap.add_argument("--tune-cache", default=None)
     a = ap.parse_args(argv)
     if a.what == "batch":
@@ -148,6 +178,8 @@ def main(argv=None):
         recs = cone()
     elif a.what == "fan":
         recs = fan()
+    elif a.what == "asd":
+        recs = [asd()]
     else:
         recs = [unet()]
     for r in recs:
