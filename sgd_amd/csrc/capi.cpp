@@ -375,7 +375,9 @@ int ensure_cone_workspace(sgdct_projector* h) {
   if (h->d_xT) return SGDCT_OK;
   const size_t ncol = (size_t)h->nx * h->ny;
   const size_t per_view = ncol * h->nv * sizeof(float);
-  size_t chunk = std::max<size_t>(1, ((size_t)1 << 30) / per_view);
+  // views per pass: as many as a 2 GiB row-profile buffer holds (16 at 256^2
+  // columns x 384 rows), a multiple of 8 so that every XCD takes whole views
+  size_t chunk = std::max<size_t>(1, ((size_t)2 << 30) / per_view);
   if (chunk >= 8) chunk -= chunk % 8;
   h->chunk = (int)std::min<size_t>(chunk, (size_t)h->np);
   hipError_t e = hipMalloc(&h->d_xT, ncol * h->nz * sizeof(float));
