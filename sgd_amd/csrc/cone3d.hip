@@ -64,8 +64,14 @@ enum ConeModel : int { CONE_B1 = 0, CONE_SF_TR = 1, CONE_SF_TT = 2 };
 // SF: lsd^2 dx dy dz / (du dv r^2) / area, SF_projector.cu:193-198, 364).
 struct AxialTrap {
   float v0, v1, v2, v3, cf;
-  float i01, i23;  // 1 / (2 (v1 - v0)), 1 / (2 (v3 - v2)); inf on a flat side, which is never used
+  float i01, i23;  // 1 / (2 (v1 - v0)), 1 / (2 (v3 - v2)); 0 on a flat side
 };
+
+// v_med3_f32: clamp of s into [lo, hi] (lo <= hi) in one instruction, with no
+// NaN quieting of the operands (every value here is finite)
+__device__ __forceinline__ float clampf(float s, float lo, float hi) { return __builtin_amdgcn_fmed3f(s, lo, hi); }
+
+__device__ __forceinline__ float inv_or_zero(float w) { return w > 0.0f ? 0.5f * __builtin_amdgcn_rcpf(w) : 0.0f; }
 
 struct ColumnAxial {
   float ka, kb;   // ldz / depth at the two x corners (SF-TR: both the centre)
@@ -102,17 +108,17 @@ __device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const Colum
   const float oz = zc * a.dz;
   if (M == CONE_B1) {
     const float w = __builtin_sqrtf(__builtin_fmaf(oz * oz, 1.0f / c.r2, 1.0f));
-    t.cf = a.cvol * w * 2.0f / ((t.v3 - t.v0) + (t.v2 - t.v1));
+    t.cf = a.cvol * w * 2.0f * __builtin_amdgcn_rcpf((t.v3 - t.v0) + (t.v2 - t.v1));
   } else {
     if (M == CONE_SF_TR) {  // a1 == b1, a2 == b2: a rectangle
       t.v1 = t.v0;
       t.v2 = t.v3;
     }
     const float area = (M == CONE_SF_TR) ? (t.v3 - t.v0) : 0.5f * ((t.v3 - t.v0) + (t.v2 - t.v1));
-    t.cf = a.fac / ((c.r2 + oz * oz) * area);
+    t.cf = a.fac * __builtin_amdgcn_rcpf((c.r2 + oz * oz) * area);
   }
-  t.i01 = 0.5f / (t.v1 - t.v0);
-  t.i23 = 0.5f / (t.v3 - t.v2);
+  t.i01 = inv_or_zero(t.v1 - t.v0);
+  t.i23 = inv_or_zero(t.v3 - t.v2);
   return t;
 }
 
@@ -123,20 +129,18 @@ __device__ __forceinline__ float row_off() {
 }
 
 // Overlap of [s, s+1) with the unit-height trapezoid (A_SF.cu:48-84 /
-// A_B1_3D.cu:30-66), segment by segment.
-// The sloped sides divide by their widths through the voxel's precomputed
-// reciprocals (a side of zero width is never entered: b2 > b1 fails).
+// A_B1_3D.cu:30-66), segment by segment: [s, s+1) clamped into each segment
+// gives the overlap's ends (equal when they do not meet).  The sloped sides
+// divide by their widths through the voxel's precomputed reciprocals, zero
+// for a side of zero width, which never overlaps.
 __device__ __forceinline__ float trap_gamma(float s, const AxialTrap& t) {
   const float e = s + 1.0f;
-  float b1 = fmaxf(s, t.v0), b2 = fminf(e, t.v1);
-  float g0 = (b2 > b1) ? (b2 - b1) * (b2 + b1 - 2.0f * t.v0) * t.i01 : 0.0f;
-  b1 = fmaxf(s, t.v1);
-  b2 = fminf(e, t.v2);
-  float g1 = fmaxf(b2 - b1, 0.0f);
-  b1 = fmaxf(s, t.v2);
-  b2 = fminf(e, t.v3);
-  float g2 = (b2 > b1) ? (b2 - b1) * (2.0f * t.v3 - b1 - b2) * t.i23 : 0.0f;
-  return g0 + g1 + g2;
+  const float s0 = clampf(s, t.v0, t.v1), e0 = clampf(e, t.v0, t.v1);
+  const float s1 = clampf(s, t.v1, t.v2), e1 = clampf(e, t.v1, t.v2);
+  const float s2 = clampf(s, t.v2, t.v3), e2 = clampf(e, t.v2, t.v3);
+  const float g0 = (e0 - s0) * (e0 + s0 - 2.0f * t.v0) * t.i01;
+  const float g2 = (e2 - s2) * (2.0f * t.v3 - s2 - e2) * t.i23;
+  return g0 + (e1 - s1) + g2;
 }
 
 template <int M>
@@ -148,8 +152,10 @@ __device__ __forceinline__ bool keep_gamma(float gm) {
 // Y[pl][c][j] = sum_iz xT[c][iz] * cfac * gamma_j.  Block: 4 waves, one
 // column each; grid (ceil(ncol / 4), views in the chunk).
 // ---------------------------------------------------------------------------
-// Dynamic LDS: per wave kAxialFields arrays of nz floats (v0..v3, x * cf,
-// i01, i23).
+// Dynamic LDS: per wave and voxel a record of kAxialFields floats (v0, v1,
+// v2, v3, i01, i23, x * cf).  The odd stride puts the records of 64
+// consecutive voxels on distinct banks, so a wave's rows (which read
+// neighbouring voxels) do not conflict.
 constexpr int kAxialFields = 7;
 
 template <int M>
@@ -157,54 +163,54 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
                                                         float* __restrict__ Y) {
   extern __shared__ float axial_lds[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  float* const L = axial_lds + (size_t)wv * kAxialFields * a.nz;  // this wave's arrays
-  float* const Lv0 = L;
-  float* const Lv1 = L + a.nz;
-  float* const Lv2 = L + 2 * a.nz;
-  float* const Lv3 = L + 3 * a.nz;
-  float* const Lxc = L + 4 * a.nz;
-  float* const Li01 = L + 5 * a.nz;
-  float* const Li23 = L + 6 * a.nz;
+  float* const R = axial_lds + (size_t)wv * kAxialFields * a.nz;  // this wave's records
   const int ncol = a.g.nx * a.g.ny;
   const int c = blockIdx.x * 4 + wv, pl = blockIdx.y;
   const bool live = c < ncol;
+  ColumnAxial ca{};
   if (live) {
-    const int ix = c % a.g.nx, iy = c / a.g.nx;
-    const ColumnAxial ca = column_axial<M>(a, a.g.views[p0 + pl], ix, iy);
+    ca = column_axial<M>(a, a.g.views[p0 + pl], c % a.g.nx, c / a.g.nx);
     for (int iz = lane; iz < a.nz; iz += 64) {
       const AxialTrap t = voxel_trap<M>(a, ca, iz);
-      Lv0[iz] = t.v0;
-      Lv1[iz] = t.v1;
-      Lv2[iz] = t.v2;
-      Lv3[iz] = t.v3;
-      Lxc[iz] = xT[(size_t)c * a.nz + iz] * t.cf;
-      Li01[iz] = t.i01;
-      Li23[iz] = t.i23;
+      float* const r = R + kAxialFields * iz;
+      r[0] = t.v0;
+      r[1] = t.v1;
+      r[2] = t.v2;
+      r[3] = t.v3;
+      r[4] = t.i01;
+      r[5] = t.i23;
+      r[6] = xT[(size_t)c * a.nz + iz] * t.cf;
     }
   }
   __syncthreads();
   if (!live) return;
   float* yrow = Y + ((size_t)pl * ncol + c) * a.nv;
+  // v3(iz) = vc + k (zc + 1/2), zc = iz - (nz - 1) / 2, with k = max(ka, kb)
+  // above vc and min(ka, kb) below; it grows with iz (ka, kb > 0: voxels in
+  // front of the source), so the first voxel reaching a row follows from that
+  // line up to rounding, and the staged values settle it exactly
+  const float ikhi = 1.0f / fmaxf(ca.ka, ca.kb), iklo = 1.0f / fminf(ca.ka, ca.kb);
+  const float zoff = 0.5f * (float)(a.nz - 1) - 0.5f;
   for (int j = lane; j < a.nv; j += 64) {
     const float fj = (float)j + row_off<M>();
-    // first voxel whose footprint reaches row j (v3 grows with iz)
-    int lo = 0, hi = a.nz;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (Lv3[mid] >= fj) hi = mid;
-      else lo = mid + 1;
-    }
+    // first voxel whose footprint reaches row j
+    const float est = __builtin_fmaf(fj - a.vc, fj >= a.vc ? ikhi : iklo, zoff);
+    int lo = (int)fminf(fmaxf(ceilf(est), 0.0f), (float)a.nz);
+    while (lo > 0 && R[kAxialFields * (lo - 1) + 3] >= fj) --lo;
+    while (lo < a.nz && R[kAxialFields * lo + 3] < fj) ++lo;
     float acc = 0.0f;
-    for (int iz = lo; iz < a.nz && Lv0[iz] < fj + 1.0f; ++iz) {
+    for (int iz = lo; iz < a.nz; ++iz) {
+      const float* const r = R + kAxialFields * iz;
+      if (!(r[0] < fj + 1.0f)) break;
       AxialTrap t;
-      t.v0 = Lv0[iz];
-      t.v1 = Lv1[iz];
-      t.v2 = Lv2[iz];
-      t.v3 = Lv3[iz];
-      t.i01 = Li01[iz];
-      t.i23 = Li23[iz];
+      t.v0 = r[0];
+      t.v1 = r[1];
+      t.v2 = r[2];
+      t.v3 = r[3];
+      t.i01 = r[4];
+      t.i23 = r[5];
       const float gm = trap_gamma(fj, t);
-      if (keep_gamma<M>(gm)) acc = __builtin_fmaf(Lxc[iz], gm, acc);
+      if (keep_gamma<M>(gm)) acc = __builtin_fmaf(r[6], gm, acc);
     }
     yrow[j] = acc;
   }
@@ -462,7 +468,7 @@ template <int M>
 static void cone_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y, float* proj,
                                hipStream_t s) {
   const int ncol = a.g.nx * a.g.ny;
-  const size_t lds = sizeof(float) * 4 * kAxialFields * a.nz;
+  const size_t lds = sizeof(float) * 4 * kAxialFields * a.nz;  // 4 waves
   axial_fwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), lds, s>>>(a, p0, xT, Y);
   // B1's in-plane weight is cheap: split the rows; SF's (four divisions) is not
   constexpr int RCW = M == CONE_B1 ? 2 : kAllRows;
