@@ -47,6 +47,7 @@
 namespace sgdct {
 
 constexpr int kMaxNz = 512;        // voxels per column staged in LDS
+constexpr int kMaxNv = 512;        // detector rows of a row profile staged in LDS
 // The in-plane kernels give each wave RCW chunks of 64 detector rows (a grid
 // dimension runs over the chunk groups): with RCW < nv / 64 more waves are in
 // flight on the row profiles, at the price of recomputing the in-plane weight
@@ -73,6 +74,14 @@ __device__ __forceinline__ float clampf(float s, float lo, float hi) { return __
 
 __device__ __forceinline__ float inv_or_zero(float w) { return w > 0.0f ? 0.5f * __builtin_amdgcn_rcpf(w) : 0.0f; }
 
+// 1 / d for a normal double d of float range: the float reciprocal (relative
+// error ~6e-8) refined by one Newton step (error ~4e-15), in place of the
+// double division's longer sequence
+__device__ __forceinline__ double recip(double d) {
+  const double r0 = (double)__builtin_amdgcn_rcpf((float)d);
+  return __builtin_fma(r0, __builtin_fma(-d, r0, 1.0), r0);
+}
+
 struct ColumnAxial {
   float ka, kb;   // ldz / depth at the two x corners (SF-TR: both the centre)
   float r2;       // |in-plane source -> centre|^2
@@ -84,12 +93,12 @@ __device__ __forceinline__ ColumnAxial column_axial(const Cone3DArgs& a, const V
   const double wy = iy * a.g.dy + vg.w0y;
   const double wxa = (ix - off) * a.g.dx + vg.w0x, wxb = (ix + off) * a.g.dx + vg.w0x;
   const double wx = ix * a.g.dx + vg.w0x;
-  // depth along the central axis = -cross(pu, X - p) / |pu|
-  const double da = -(vg.pux * wy - vg.puy * wxa) / a.du;
-  const double db = -(vg.pux * wy - vg.puy * wxb) / a.du;
+  // depth along the central axis = -cross(pu, X - p) / |pu|, so
+  // ldz / depth = ldz |pu| / cross(p - X, pu) with |pu| = du
+  const double num = a.ldz * a.du;
   ColumnAxial c;
-  c.ka = (float)(a.ldz / da);
-  c.kb = (float)(a.ldz / db);
+  c.ka = (float)(num * recip(vg.puy * wxa - vg.pux * wy));
+  c.kb = (float)(num * recip(vg.puy * wxb - vg.pux * wy));
   c.r2 = (float)(wx * wx + wy * wy);
   return c;
 }
@@ -396,24 +405,50 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
 
 // ---------------------------------------------------------------------------
 // xT[c][iz] (+)= sum over the chunk's views of cfac * sum_j gamma_j Yb[c][j].
-// Wave per column, lanes over voxels; grid ceil(ncol / 4).
+// Wave per column, lanes over voxels; grid ceil(ncol / 4).  Each view's row
+// profile Yb[c][*] is copied into the wave's LDS with coalesced loads (the
+// next view's loads are in flight meanwhile); the voxels then gather their
+// rows from LDS.
 // ---------------------------------------------------------------------------
+// Orders a wave's LDS accesses across lanes: LDS operations of one wave are
+// performed in program order, so a compiler barrier at wavefront scope suffices.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int M>
 __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, int nview, const float* __restrict__ Yb,
                                                         float* __restrict__ xT, int accumulate) {
+  __shared__ float prof_lds[4][kMaxNv];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ncol = a.g.nx * a.g.ny;
   const int c = blockIdx.x * 4 + wv;
-  if (c >= ncol) return;
+  if (c >= ncol) return;  // waves share no LDS and meet at no block barrier
+  float* const yl = prof_lds[wv];
   const int ix = c % a.g.nx, iy = c / a.g.nx;
-  constexpr int kSlots = kMaxNz / 64;
+  constexpr int kSlots = kMaxNz / 64, kRowSlots = kMaxNv / 64;
   double acc[kSlots];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) acc[s] = 0.0;
+  float pre[kRowSlots];
+  auto fetch = [&](int pl) {
+    const float* yrow = Yb + ((size_t)pl * ncol + c) * a.nv;
+#pragma unroll
+    for (int k = 0; k < kRowSlots; ++k)
+      if (k * 64 < a.nv) pre[k] = (k * 64 + lane < a.nv) ? yrow[k * 64 + lane] : 0.0f;
+  };
+  fetch(0);
   const float lastj = (float)(a.nv - 1);
   for (int pl = 0; pl < nview; ++pl) {
+    wave_lds_sync();  // the previous view's reads come first
+#pragma unroll
+    for (int k = 0; k < kRowSlots; ++k)
+      if (k * 64 + lane < a.nv) yl[k * 64 + lane] = pre[k];
+    wave_lds_sync();
+    if (pl + 1 < nview) fetch(pl + 1);
     const ColumnAxial ca = column_axial<M>(a, a.g.views[p0 + pl], ix, iy);
-    const float* yrow = Yb + ((size_t)pl * ncol + c) * a.nv;
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
       const int iz = s * 64 + lane;
@@ -425,7 +460,7 @@ __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, in
       float sum = 0.0f;
       for (int j = jlo; j <= jhi; ++j) {
         const float gm = trap_gamma((float)j + ro, t);
-        if (keep_gamma<M>(gm)) sum = __builtin_fmaf(gm, yrow[j], sum);
+        if (keep_gamma<M>(gm)) sum = __builtin_fmaf(gm, yl[j], sum);
       }
       acc[s] += (double)(t.cf * sum);
     }
