@@ -312,34 +312,65 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
   double acc[RCW];
 #pragma unroll
   for (int k = 0; k < RCW; ++k) acc[k] = 0.0;
+  // Row-profile reads of a column: every lane loads (rows past nv read the
+  // last row and are never stored), so the loads need no per-lane branch.
   const float* ybase = Y + (size_t)pl * ncol * a.nv + kc0 * 64 + lane;
+  const int jmax = a.nv - 1 - kc0 * 64 - lane;  // offset of the last row
+  auto load_col = [&](int m, int n, float* y) {
+    const float* yc = ybase + (size_t)(m * smaj + n * smin) * a.nv;
+#pragma unroll
+    for (int k = 0; k < RCW; ++k)
+      if ((kc0 + k) * 64 < a.nv) y[k] = yc[min(k * 64, jmax)];
+  };
+  // strip range of line m (empty when n0 > n1)
+  auto line_range = [&](int m, int& n0, int& n1) {
+    n0 = (int)fmax(ceil(fma((double)m, lo1, lo0)), 0.0);
+    n1 = (int)fmin(floor(fma((double)m, hi1, hi0)), (double)(nmin - 1));
+  };
+  // Lines are walked in order; within a line the next column's profile is
+  // loaded while the current column's weight is computed, and the last
+  // column of a line loads the first of the next line.
+  float ycur[RCW], ynext[RCW];
+#pragma unroll
+  for (int k = 0; k < RCW; ++k) ycur[k] = ynext[k] = 0.0f;
+  int n0, n1;
+  line_range(m_lo, n0, n1);
+  if (n0 <= n1) load_col(m_lo, n0, ycur);
   for (int m = m_lo; m <= m_hi; ++m) {
-    const int n0 = (int)fmax(ceil(fma((double)m, lo1, lo0)), 0.0);
-    const int n1 = (int)fmin(floor(fma((double)m, hi1, hi0)), (double)(nmin - 1));
-    if (n0 > n1) continue;
-    float ta = (float)fma((double)n0, an, fma((double)m, am, a0));
-    float tb = (float)fma((double)n0, bn, fma((double)m, bm, b0));
-    float accl[RCW];
+    int q0 = 0, q1 = -1;  // next line's range
+    if (m < m_hi) line_range(m + 1, q0, q1);
+    if (n0 <= n1) {
+      float ta = (float)fma((double)n0, an, fma((double)m, am, a0));
+      float tb = (float)fma((double)n0, bn, fma((double)m, bm, b0));
+      float accl[RCW];
 #pragma unroll
-    for (int k = 0; k < RCW; ++k) accl[k] = 0.0f;
-    for (int n = n0; n <= n1; ++n) {
-      float w;
-      if (M == CONE_B1) {
-        w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4),
-                      s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4), ta, tb) * rec.scale;
-      } else {
-        const int ix = rows ? n : m, iy = rows ? m : n;
-        w = sf_cell_weight(vg, g, u, ix, iy);
+      for (int k = 0; k < RCW; ++k) accl[k] = 0.0f;
+      for (int n = n0; n <= n1; ++n) {
+        if (n < n1) load_col(m, n + 1, ynext);
+        else if (q0 <= q1) load_col(m + 1, q0, ynext);
+        float w;
+        if (M == CONE_B1) {
+          w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.k2, rec.k4),
+                        s_b1(tb, rec.apb, rec.b, rec.a, rec.k2, rec.k4), ta, tb) * rec.scale;
+        } else {
+          const int ix = rows ? n : m, iy = rows ? m : n;
+          w = sf_cell_weight(vg, g, u, ix, iy);
+        }
+#pragma unroll
+        for (int k = 0; k < RCW; ++k) {
+          accl[k] = __builtin_fmaf(w, ycur[k], accl[k]);
+          ycur[k] = ynext[k];
+        }
+        ta += dta;
+        tb += dtb;
       }
-      const float* yc = ybase + (size_t)(m * smaj + n * smin) * a.nv;
 #pragma unroll
-      for (int k = 0; k < RCW; ++k)
-        if ((kc0 + k) * 64 + lane < a.nv) accl[k] = __builtin_fmaf(w, yc[k * 64], accl[k]);
-      ta += dta;
-      tb += dtb;
+      for (int k = 0; k < RCW; ++k) acc[k] += (double)accl[k];
+    } else if (q0 <= q1) {
+      load_col(m + 1, q0, ycur);
     }
-#pragma unroll
-    for (int k = 0; k < RCW; ++k) acc[k] += (double)accl[k];
+    n0 = q0;
+    n1 = q1;
   }
   float* out = proj + (size_t)v * a.nv * g.nu + u;
 #pragma unroll
@@ -379,8 +410,23 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
   float acc[RCW];
 #pragma unroll
   for (int k = 0; k < RCW; ++k) acc[k] = 0.0f;
+  // every lane loads (rows past nv read the last row and are never stored);
+  // the next bin's rows are loaded while the current bin's weight is computed
+  const float* rows0 = projT + (size_t)pl * g.nu * a.nv + kc0 * 64 + lane;
+  const int jmax = a.nv - 1 - kc0 * 64 - lane;
+  auto load_bin = [&](int u, float* y) {
+    const float* row = rows0 + (size_t)u * a.nv;
+#pragma unroll
+    for (int k = 0; k < RCW; ++k)
+      if ((kc0 + k) * 64 < a.nv) y[k] = row[min(k * 64, jmax)];
+  };
+  float ycur[RCW], ynext[RCW];
+#pragma unroll
+  for (int k = 0; k < RCW; ++k) ycur[k] = ynext[k] = 0.0f;
+  load_bin(ulo, ycur);
   float na = (float)fma((double)ulo, be, n0);
   for (int u = ulo; u <= uhi; ++u) {
+    if (u < uhi) load_bin(u + 1, ynext);
     const float nb = na + bef;
     float w;
     if (M == CONE_B1) {
@@ -391,10 +437,11 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
     } else {
       w = sf_cell_weight(vg, g, u, ix, iy);
     }
-    const float* row = projT + ((size_t)pl * g.nu + u) * a.nv + kc0 * 64 + lane;
 #pragma unroll
-    for (int k = 0; k < RCW; ++k)
-      if ((kc0 + k) * 64 + lane < a.nv) acc[k] = __builtin_fmaf(w, row[k * 64], acc[k]);
+    for (int k = 0; k < RCW; ++k) {
+      acc[k] = __builtin_fmaf(w, ycur[k], acc[k]);
+      ycur[k] = ynext[k];
+    }
     na = nb;
   }
   float* out = Yb + ((size_t)pl * ncol + c) * a.nv + kc0 * 64 + lane;
