@@ -84,6 +84,10 @@ __device__ __forceinline__ double hex_reach(double gx, double gy, double dt) {
   return dt * fmax(fabs(gy), 0.8660254037844386 * fabs(gx) + 0.5 * fabs(gy));
 }
 
+__device__ __forceinline__ float hex_reach_f(float gx, float gy, float dt) {
+  return dt * fmaxf(__builtin_fabsf(gy), __builtin_fmaf(0.8660254f, __builtin_fabsf(gx), 0.5f * __builtin_fabsf(gy)));
+}
+
 struct HexArgs {
   Proj2DArgs g;  // nx = nc, ny = nl, dx = dxh, dy = dyh
   double dt;
@@ -219,15 +223,20 @@ __global__ __launch_bounds__(256) void hex_bwd_kernel(HexArgs ha, const float* _
         const double wx = (ic + sh) * g.dx + vg.w0x, wy = il * g.dy + vg.w0y;
         const double n0 = vg.e0x * wy - vg.e0y * wx;
         const double be = vg.pux * wy - vg.puy * wx;
-        const double kc = -n0 / be;
-        // k*(vertex) - kc = -(d . G) / (beta + d . grad beta), G = grad N_0 + kc grad beta
-        const double gx = -(vg.e0y + kc * vg.puy), gy = vg.e0x + kc * vg.pux;
-        const double hw = hex_reach(gx, gy, ha.dt) / (fabs(be) - hex_reach(-vg.puy, vg.pux, ha.dt));
-        const double last = (double)(g.nu - 1);
-        const int ulo = (int)fmin(fmax(floor(kc - hw), 0.0), last);
-        const int uhi = (int)fmin(fmax(floor(kc + hw), 0.0), last);
-        float na = (float)(n0 + (double)ulo * be);
         const float bef = (float)be;
+        // bin range in FP32 (it only bounds the bins visited; a bin gained or
+        // lost at a rounding boundary carries a vanishing footprint)
+        const float kc = -(float)n0 * __builtin_amdgcn_rcpf(bef);
+        // k*(vertex) - kc = -(d . G) / (beta + d . grad beta), G = grad N_0 + kc grad beta
+        const float gx = -__builtin_fmaf(kc, (float)vg.puy, (float)vg.e0y);
+        const float gy = __builtin_fmaf(kc, (float)vg.pux, (float)vg.e0x);
+        const float dt = (float)ha.dt;
+        const float hw = hex_reach_f(gx, gy, dt) *
+                         __builtin_amdgcn_rcpf(__builtin_fabsf(bef) - hex_reach_f(-(float)vg.puy, (float)vg.pux, dt));
+        const float last = (float)(g.nu - 1);
+        const int ulo = (int)fminf(fmaxf(floorf(kc - hw), 0.0f), last);
+        const int uhi = (int)fminf(fmaxf(floorf(kc + hw), 0.0f), last);
+        float na = (float)(n0 + (double)ulo * be);
         const int wlo = win_lo[j], wn = win_n[j];
         float accv = 0.0f;
         for (int u = ulo; u <= uhi; ++u) {
