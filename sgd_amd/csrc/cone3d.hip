@@ -552,8 +552,11 @@ static void cone_forward_chunk(const Cone3DArgs& a, int p0, int nview, const flo
   const int ncol = a.g.nx * a.g.ny;
   const size_t lds = sizeof(float) * 4 * kAxialFields * a.nz;  // 4 waves
   axial_fwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), lds, s>>>(a, p0, xT, Y);
-  // B1's in-plane weight is cheap: split the rows; SF's (four divisions) is not
-  constexpr int RCW = M == CONE_B1 ? 2 : kAllRows;
+  // every row in one wave: with the next column's profile loaded during the
+  // weight computation, splitting the rows over more waves (each computing
+  // the weights again) no longer pays (B1, 16 views of 256^2 x 384: 2 chunks
+  // of 64 rows per wave 67 ms per call, 3 chunks 61 ms, all 55 ms)
+  constexpr int RCW = kAllRows;
   const int groups = (a.nrc + RCW - 1) / RCW;
   conv_fwd_kernel<M, RCW><<<dim3(((a.g.nu + 3) / 4) * nview, groups), dim3(256), 0, s>>>(a, p0, nview, Y, proj);
 }
