@@ -13,13 +13,18 @@ struct Trap {
   float q0, q1, q2, q3, ir, iff;  // sorted corners, 1/(2(q1-q0)), 1/(2(q3-q2))
 };
 
+// min / max of finite values as v_med3_f32 against -inf / +inf: exact, one
+// instruction, and no NaN quieting of the operands (fminf / fmaxf add it)
+__device__ __forceinline__ float min2f(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -__builtin_inff()); }
+__device__ __forceinline__ float max2f(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, __builtin_inff()); }
+
 __device__ __forceinline__ void sort4(float& a, float& b, float& c, float& d) {
   float t;
-  t = fminf(a, b); b = fmaxf(a, b); a = t;
-  t = fminf(c, d); d = fmaxf(c, d); c = t;
-  t = fminf(a, c); c = fmaxf(a, c); a = t;
-  t = fminf(b, d); d = fmaxf(b, d); b = t;
-  t = fminf(b, c); c = fmaxf(b, c); b = t;
+  t = min2f(a, b); b = max2f(a, b); a = t;
+  t = min2f(c, d); d = max2f(c, d); c = t;
+  t = min2f(a, c); c = max2f(a, c); a = t;
+  t = min2f(b, d); d = max2f(b, d); b = t;
+  t = min2f(b, c); c = max2f(b, c); b = t;
 }
 
 __device__ __forceinline__ Trap make_trap(float q0, float q1, float q2, float q3) {
@@ -46,9 +51,11 @@ __device__ __forceinline__ float trap_gamma(const Trap& t, float s1, float s2) {
   return g;
 }
 
+// dx / max(|cos t|, |sin t|) of the ray through (wx, wy) from the source;
+// v_sqrt_f32 (1 ulp) instead of the correctly rounded square-root sequence
 __device__ __forceinline__ float sf_weight(float dx, float wx, float wy) {
   float r2 = wx * wx + wy * wy;
-  return dx * __builtin_sqrtf(r2) * __builtin_amdgcn_rcpf(fmaxf(fabsf(wx), fabsf(wy)));
+  return dx * __builtin_amdgcn_sqrtf(r2) * __builtin_amdgcn_rcpf(fmaxf(fabsf(wx), fabsf(wy)));
 }
 
 // A: bin-driven gather (same strip walk as fwd_bin_kernel, corner rule h = 1/2).
@@ -86,6 +93,10 @@ __device__ __forceinline__ void sf_fwd_wave(const Proj2DArgs& g, const float* __
   const float nx_ = 0.5f * (float)acx, ny_ = 0.5f * (float)acy;
   const float bx_ = 0.5f * (float)bcxx, by_ = 0.5f * (float)bcyy;
   const float fdx = (float)g.dx;
+  // corner offsets along the walk (w) and across it (c): the corners on the
+  // leading side (+w) of one pixel are the trailing side (-w) of the next
+  const float w_n = rows ? nx_ : ny_, c_n = rows ? ny_ : nx_;
+  const float w_b = rows ? bx_ : by_, c_b = rows ? by_ : bx_;
 
   int m_lo = 0, m_hi = nmaj - 1;
   if (ma != 0.0 && mb != 0.0) {
@@ -120,13 +131,16 @@ __device__ __forceinline__ void sf_fwd_wave(const Proj2DArgs& g, const float* __
     float wx = (float)(ixs * g.dx + vg.w0x), wy = (float)(iys * g.dy + vg.w0y);
     const float* p = img + (size_t)m * smaj + (size_t)n0 * smin;
     float accl = 0.0f;
+    // corner projections relative to edge u, -N_u(c) / beta(c), of the
+    // trailing corners of the line's first pixel
+    float t1 = -(nu_ - w_n - c_n) * __builtin_amdgcn_rcpf(be - w_b - c_b);
+    float t2 = -(nu_ - w_n + c_n) * __builtin_amdgcn_rcpf(be - w_b + c_b);
     for (int n = n0; n <= n1; ++n) {
-      // corner projections relative to edge u: -N_u(c) / beta(c)
-      const float q0 = -(nu_ - nx_ - ny_) * __builtin_amdgcn_rcpf(be - bx_ - by_);
-      const float q1 = -(nu_ - nx_ + ny_) * __builtin_amdgcn_rcpf(be - bx_ + by_);
-      const float q2 = -(nu_ + nx_ - ny_) * __builtin_amdgcn_rcpf(be + bx_ - by_);
-      const float q3 = -(nu_ + nx_ + ny_) * __builtin_amdgcn_rcpf(be + bx_ + by_);
-      const Trap t = make_trap(q0, q1, q2, q3);
+      const float l1 = -(nu_ + w_n - c_n) * __builtin_amdgcn_rcpf(be + w_b - c_b);
+      const float l2 = -(nu_ + w_n + c_n) * __builtin_amdgcn_rcpf(be + w_b + c_b);
+      const Trap t = make_trap(t1, t2, l1, l2);
+      t1 = l1;
+      t2 = l2;
       const float gm = trap_gamma(t, 0.0f, 1.0f);
       const float val = (gm > 1e-7f) ? gm * sf_weight(fdx, wx, wy) : 0.0f;
       accl = __builtin_fmaf(val, *p, accl);
