@@ -225,13 +225,13 @@ void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, vo
       if (b > 1e-7) {
         sc = std::pow(24.0 * a * a * b * b, -0.25);  // c^(1/4)
         const double bs = b * sc;
-        r.k2 = (float)(12.0 * bs * bs);
-        r.k4 = (float)(2.0 * bs * bs * bs * bs);
+        r.nk2 = (float)(-12.0 * bs * bs);
+        r.h4 = (float)(0.5 - 2.0 * bs * bs * bs * bs);
       } else {  // 1tri1box limit, spline_conv.hh:393-405
         b = 0.0;
         sc = 1.0 / a;
-        r.k2 = 0.5f;
-        r.k4 = 0.0f;
+        r.nk2 = -0.5f;
+        r.h4 = 0.5f;
       }
       r.ka = (float)(ka * sc);
       r.kb = (float)(kb * sc);

@@ -33,10 +33,11 @@ namespace sgdct {
 //   conv = (S(tb) - S(ta)) / (tb - ta) = s (S(tb') - S(ta')) / (tb' - ta').
 // Traits<B1>::conv returns the quotient without the factor s; A applies
 // `scale` once per bin, A^T stages it into the sinogram value.
+// The record holds -k2 and 1/2 - k4, so that 1/2 - Q comes out of one FMA.
 struct B1Bin {
   float ka, kb;      // ta' = ka * N_u(pixel), tb' = kb * N_{u+1}(pixel)
   float apb, b, a;   // a' + b', b', a'
-  float k2, k4;      // see above
+  float nk2, h4;     // -k2, 1/2 - k4
   float scale;       // s (A^T's LDS copy holds sinogram value * s here)
 };
 
@@ -56,19 +57,20 @@ __device__ __forceinline__ float copysign_f(float mag, float sgn) {
   return __builtin_copysignf(mag, sgn);
 }
 
-__device__ __forceinline__ float s_b1(float t, float apb, float b, float a, float k2, float k4) {
+// S(t) in the scaled offset.  With m = max(y - b', 0): y >= 2b' exactly when
+// m >= b' (b' >= 0), and there m = y - b', so both branches of 1/2 - Q use m^2.
+__device__ __forceinline__ float s_b1(float t, float apb, float b, float a, float nk2, float h4) {
   float x = __builtin_fabsf(t);
   float y = __builtin_fmaxf(apb - x, 0.0f);
-  float v = y - b;
-  float qa = __builtin_fmaf(k2, v * v, k4);
-  float y2 = y * y;
-  float m = __builtin_fmaxf(v, 0.0f);
+  float m = __builtin_fmaxf(y - b, 0.0f);
   float m2 = m * m;
-  float qb = __builtin_fmaf(-2.0f, m2 * m2, y2 * y2);
-  float q = (v >= b) ? qa : qb;
+  float y2 = y * y;
+  float ha = __builtin_fmaf(nk2, m2, h4);                                       // 1/2 - k2 m^2 - k4
+  float hb = __builtin_fmaf(2.0f, m2 * m2, __builtin_fmaf(-y2, y2, 0.5f));      // 1/2 - y^4 + 2 m^4
+  float h = (m >= b) ? ha : hb;
   float e = __builtin_fmaxf(y - a, 0.0f);
   float e2 = e * e;
-  float s = __builtin_fmaf(2.0f, e2 * e2, 0.5f - q);
+  float s = __builtin_fmaf(2.0f, e2 * e2, h);
   return copysign_f(s, t);
 }
 

@@ -21,8 +21,8 @@ template <> struct Traits<BASIS_B1> {
   using Bin = B1Bin;
   // (S(tb') - S(ta')) / (tb' - ta'); times scale(r) it is the footprint.
   __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
-    float sa = s_b1(ta, r.apb, r.b, r.a, r.k2, r.k4);
-    float sb = s_b1(tb, r.apb, r.b, r.a, r.k2, r.k4);
+    float sa = s_b1(ta, r.apb, r.b, r.a, r.nk2, r.h4);
+    float sb = s_b1(tb, r.apb, r.b, r.a, r.nk2, r.h4);
     return mean_over(sa, sb, ta, tb);
   }
   __host__ __device__ static __forceinline__ float scale(const Bin& r) { return r.scale; }
