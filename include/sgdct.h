@@ -127,22 +127,23 @@ void sgdct_destroy(sgdct_projector *h);
  * (cone beam: volume (nz, ny, nx) and projections (np, nv, nu)), both device
  * pointers; `batch` images / sinograms back to back; `stream` is
  * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.  Cone-beam
- * handles, and B0 / B1 / SF handles on launches they split (below), keep per-handle
+ * handles, and fan-beam handles on launches they split (below), keep per-handle
  * workspace: calls on one handle must not overlap (issue them on one stream).
  * Returns SGDCT_OK or a negative code. */
 int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
 
-/* B0 / B1 / SF launch shape (no reference counterpart).  A splits every bin's
- * pixel walk into `fwd_split` line segments and A^T the views into
- * `bwd_split` groups, each writing FP64 partial sums that a second kernel adds
+/* Fan-beam launch shape (no reference counterpart).  A splits every bin's
+ * pixel walk into `fwd_split` line segments (B0 / B1 / SF; hexagonal handles
+ * take 0 or 1) and A^T the views into `bwd_split` groups (every fan-beam
+ * basis), each writing FP64 partial sums that a second kernel adds
  * in a fixed order (results are deterministic, independent of the batch size,
  * and agree across splits to FP64 rounding).  0 = automatic: chosen at
  * creation from one image's launch size (a launch of few waves splits, up to
- * 16 ways); 1..16 = fixed.  The partial-sum workspace (up to 2 images per
- * launch; larger batches take several launches) is allocated at creation and
- * by this call, never inside a projection.  Returns SGDCT_OK, SGDCT_EINVAL or
- * SGDCT_ENOMEM. */
+ * 16 ways); 1..16 = fixed.  The partial-sum workspace (2 images per launch,
+ * or up to 16 while it stays within 64 MiB; larger batches take several
+ * launches) is allocated at creation and by this call, never inside a
+ * projection.  Returns SGDCT_OK, SGDCT_EINVAL or SGDCT_ENOMEM. */
 int sgdct_split_config(sgdct_projector *h, int fwd_split, int bwd_split);
 
 /* Shape query: out[0..3] = nx, ny, np, nu (hexagonal: nc, nl, np, nu);

@@ -31,7 +31,7 @@ size_t hex_bin_size(int basis);
 hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const float* img, float* sino, int batch,
                               hipStream_t s);
 hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const float* sino, float* img, int batch,
-                               hipStream_t s);
+                               int ngroup, double* part, hipStream_t s);
 hipError_t launch_hex_count(const Proj2DArgs& g, double dt, unsigned long long* out, hipStream_t s);
 hipError_t launch_tv_grad(const float* x, float* g, int nx, int ny, int batch, float eps, hipStream_t s);
 hipError_t launch_resid(const float* b, const float* ax, const float* w, float* r, size_t n, size_t per, hipStream_t s);
@@ -202,8 +202,7 @@ int auto_bwd_split(const sgdct_projector* h);
 int ensure_part(sgdct_projector* h, size_t doubles);
 void release_part(sgdct_projector* h);
 size_t part_need(const sgdct_projector* h);
-// images per launch the partial-sum workspace holds
-constexpr int kPartImages = 2;
+int part_images(const sgdct_projector* h);
 
 // Per-(view, bin) constants, FP64 on the host, stored FP32 like the
 // reference's per-bin vectors (layers/A_B1.cu:429-478, A_B0.cu:338-388).
@@ -257,6 +256,10 @@ void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, vo
 }
 
 bool is_hex(int basis) { return basis == BASIS_H0 || basis == BASIS_HL; }
+// A's walk splits into line segments for the square-lattice bases; A^T's
+// views split into groups for every fan-beam basis
+bool splits_fwd(int basis) { return basis == BASIS_B0 || basis == BASIS_B1 || basis == BASIS_SF; }
+bool splits_bwd(int basis) { return splits_fwd(basis) || is_hex(basis); }
 
 // ushift moves the bin edges by ushift bins: 0.5 gives cells centred on
 // integer detector coordinates (layers/SF_projector.cu:209-216).
@@ -329,10 +332,10 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   a.nchunk = (nu + 63) / 64;
   // 2 bins of slack for FP64 contraction differences between host and device
   if (basis == BASIS_B0 || basis == BASIS_B1) h->bwd_lds = max_tile_window(a, views.data()) <= kMaxWindow - 2;
-  if (basis == BASIS_B0 || basis == BASIS_B1 || basis == BASIS_SF) {
-    h->fwd_auto = auto_fwd_split(h);
+  if (splits_bwd(basis)) {
+    if (splits_fwd(basis)) h->fwd_auto = auto_fwd_split(h);
     h->bwd_auto = auto_bwd_split(h);
-    if (ensure_part(h, kPartImages * part_need(h)) != SGDCT_OK) {
+    if (ensure_part(h, part_images(h) * part_need(h)) != SGDCT_OK) {
       delete h;
       return nullptr;
     }
@@ -468,7 +471,19 @@ size_t part_need(const sgdct_projector* h) {
   return std::max(fs > 1 ? (size_t)fs * h->np * h->nu : 0, bs > 1 ? (size_t)bs * h->nx * h->ny : 0);
 }
 
-bool splits_launches(int basis) { return basis == BASIS_B0 || basis == BASIS_B1 || basis == BASIS_SF; }
+// Images per split launch: at least 2, and more (up to 16) while the
+// workspace stays within kPartBytes.  A batch runs as launches of this many
+// images; launches of few small images lose a larger share to their tails
+// (hexagonal A^T at 296 x 256 points, batch 64: 2 images per launch 60.9 ms,
+// 8 images 58.7 ms).
+constexpr size_t kPartBytes = (size_t)64 << 20;
+
+int part_images(const sgdct_projector* h) {
+  const size_t per = part_need(h) * sizeof(double);
+  if (per == 0) return 2;
+  return (int)std::max<size_t>(2, std::min<size_t>(16, kPartBytes / per));
+}
+
 
 hipError_t launch_fwd2d(const sgdct_projector* h, const float* img, float* sino, int batch, int nseg, double* part,
                         hipStream_t s) {
@@ -479,6 +494,7 @@ hipError_t launch_fwd2d(const sgdct_projector* h, const float* img, float* sino,
 hipError_t launch_bwd2d(const sgdct_projector* h, const float* sino, float* img, int batch, int ngroup, double* part,
                         hipStream_t s) {
   if (h->basis == BASIS_SF) return launch_sf_backward(h->args, sino, img, batch, ngroup, part, s);
+  if (is_hex(h->basis)) return launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, ngroup, part, s);
   return launch_backward(h->basis, h->args, h->bwd_lds, sino, img, batch, ngroup, part, s);
 }
 
@@ -505,9 +521,7 @@ int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hi
 int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
   if (is_cone(h->basis)) return cone_backward(h, sino, img, batch, s);
   hipError_t e = hipSuccess;
-  if (is_hex(h->basis)) {
-    e = launch_hex_backward(h->basis, h->args, h->dt, sino, img, batch, s);
-  } else {
+  {
     const int ng = h->bwd_split ? h->bwd_split : h->bwd_auto;
     const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)ng * ni;
     if (ng == 1) {
@@ -1046,11 +1060,13 @@ int sgdct_batch_graph_launch(sgdct_batch* b, int direction, const float* in, flo
 int sgdct_split_config(sgdct_projector* h, int fwd_split, int bwd_split) {
   if (!h || fwd_split < 0 || fwd_split > kMaxSplit || bwd_split < 0 || bwd_split > kMaxSplit)
     return fail(SGDCT_EINVAL, "split_config: splits must be 0 (automatic) or 1..16");
-  if (!splits_launches(h->basis)) return fail(SGDCT_EINVAL, "split_config: B0 / B1 / SF handles only");
+  if (!splits_bwd(h->basis)) return fail(SGDCT_EINVAL, "split_config: fan-beam handles only");
+  if (!splits_fwd(h->basis) && fwd_split > 1)
+    return fail(SGDCT_EINVAL, "split_config: hexagonal handles split A^T only");
   const int fs = h->fwd_split, bs = h->bwd_split;
   h->fwd_split = fwd_split;
   h->bwd_split = bwd_split;
-  if (int rc = ensure_part(h, kPartImages * part_need(h))) {
+  if (int rc = ensure_part(h, part_images(h) * part_need(h))) {
     h->fwd_split = fs;
     h->bwd_split = bs;
     return rc;

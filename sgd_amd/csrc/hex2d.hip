@@ -23,6 +23,8 @@
 
 namespace sgdct {
 
+hipError_t launch_sum_parts(const double* part, float* out, size_t n, int nparts, hipStream_t s);
+
 // Rhombus (va, vb) of H0: trapezoid box_p * box_q centred at c = (va + vb)/2.
 struct H0Bin {
   float ka, kb;
@@ -165,9 +167,13 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
 }
 
 // A^T: lattice-point-driven gather, 16 x 16 point tiles, LDS bin windows.
+// grid (tiles, tiles, batch * ngroup): block z = b * ngroup + k takes views
+// [k vpg, (k+1) vpg); with ngroup > 1 it writes its FP64 sum to
+// part[(k * batch + b) * nc * nl + point], added in group order afterwards.
 template <int HB>
 __global__ __launch_bounds__(256) void hex_bwd_kernel(HexArgs ha, const float* __restrict__ sino,
-                                                      float* __restrict__ img) {
+                                                      float* __restrict__ img, int ngroup, int vpg,
+                                                      double* __restrict__ part) {
   using T = HexTraits<HB>;
   using Bin = typename T::Bin;
   constexpr int BIN4 = sizeof(Bin) / 16;
@@ -178,16 +184,19 @@ __global__ __launch_bounds__(256) void hex_bwd_kernel(HexArgs ha, const float* _
   const Proj2DArgs& g = ha.g;
   const int ic0 = blockIdx.x * kTile, il0 = blockIdx.y * kTile;
   const int ic = ic0 + (threadIdx.x & (kTile - 1)), il = il0 + threadIdx.x / kTile;
-  sino += (size_t)blockIdx.z * g.np * g.nu;
-  img += (size_t)blockIdx.z * g.nx * g.ny;
+  const int k = blockIdx.z % ngroup, b = blockIdx.z / ngroup, batch = gridDim.z / ngroup;
+  const size_t npt = (size_t)g.nx * g.ny;
+  sino += (size_t)b * g.np * g.nu;
+  img += b * npt;
+  const int v_begin = k * vpg, v_end = min(g.np, v_begin + vpg);
   const bool inside = ic < g.nx && il < g.ny;
   const bool valid = inside && !((il & 1) && ic == g.nx - 1);
   const Bin* bins = reinterpret_cast<const Bin*>(g.bins);
   const double sh = (il & 1) ? 0.5 : 0.0;
 
   double acc = 0.0;
-  for (int v0 = 0; v0 < g.np; v0 += kViewChunk) {
-    const int nv = min(kViewChunk, g.np - v0);
+  for (int v0 = v_begin; v0 < v_end; v0 += kViewChunk) {
+    const int nv = min(kViewChunk, v_end - v0);
     if ((int)threadIdx.x < nv) {
       // window: the tile's bounding box widened by one circumradius
       const ViewGeo& vg = g.views[v0 + threadIdx.x];
@@ -261,7 +270,11 @@ __global__ __launch_bounds__(256) void hex_bwd_kernel(HexArgs ha, const float* _
     }
     __syncthreads();
   }
-  if (inside) img[(size_t)il * g.nx + ic] = valid ? (float)(acc * g.dx * g.dy) : 0.0f;
+  if (inside) {
+    const double val = valid ? acc * g.dx * g.dy : 0.0;
+    if (ngroup > 1) part[(size_t)k * batch * npt + b * npt + (size_t)il * g.nx + ic] = val;
+    else img[(size_t)il * g.nx + ic] = (float)val;
+  }
 }
 
 // Evaluation count (clamped six-vertex ranges).
@@ -354,11 +367,14 @@ hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const f
 }
 
 hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const float* sino, float* img, int batch,
-                               hipStream_t s) {
+                               int ngroup, double* part, hipStream_t s) {
+  if (ngroup < 1 || (ngroup > 1 && !part)) return hipErrorInvalidValue;
   HexArgs ha{g, dt};
-  dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, batch), block(256);
-  if (basis == BASIS_H0) hex_bwd_kernel<BASIS_H0><<<grid, block, 0, s>>>(ha, sino, img);
-  else hex_bwd_kernel<BASIS_HL><<<grid, block, 0, s>>>(ha, sino, img);
+  const int vpg = (g.np + ngroup - 1) / ngroup;
+  dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, batch * ngroup), block(256);
+  if (basis == BASIS_H0) hex_bwd_kernel<BASIS_H0><<<grid, block, 0, s>>>(ha, sino, img, ngroup, vpg, part);
+  else hex_bwd_kernel<BASIS_HL><<<grid, block, 0, s>>>(ha, sino, img, ngroup, vpg, part);
+  if (ngroup > 1) return launch_sum_parts(part, img, (size_t)batch * g.nx * g.ny, ngroup, s);
   return hipGetLastError();
 }
 
