@@ -174,6 +174,14 @@ int sgdct_sirt_update(float *x, const float *c, const float *v, float lam, int p
 int sgdct_axpy(float *x, const float *g, float a, long long n, void *stream);
 /* x[k] += a[k / per] g[k], a a device array of n / per coefficients */
 int sgdct_axpy_batch(float *x, const float *g, const float *a, long long n, long long per, void *stream);
+/* `nsteps` steps of ASD-POCS's normalised TV descent on a batch of images,
+ * with no host synchronisation between them: g = tv_gradient(x), then
+ * x[b] -= step[b] * g[b] / ||g[b]|| for each image with a nonzero gradient
+ * (||g[b]|| summed in FP64 in a fixed order; deterministic).  step: `batch`
+ * FP64 values on the device (0 leaves an image unchanged); g: device space
+ * for the gradients (nx * ny * batch floats); work: 64 * batch doubles. */
+int sgdct_tv_descent(float *x, float *g, double *work, const double *step, int nx, int ny, int batch,
+                     float eps, int nsteps, void *stream);
 
 /* ---- batched slices (no reference counterpart: the reference projects one
  * slice per call and launches one kernel per view, layers/A_B1.cu:512-541) --

@@ -100,6 +100,7 @@ def lib():
         "sgdct_nn_fill": [_v, _ll, _f, _v],
         "sgdct_nn_to_bf16": [_v, _v, _ll, _v],
         "sgdct_axpy_batch": [_v, _v, _v, _ll, _ll, _v],
+        "sgdct_tv_descent": [_v, _v, _v, _v, _i, _i, _i, _f, _i, _v],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

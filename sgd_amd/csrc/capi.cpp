@@ -34,6 +34,8 @@ hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const 
                                int ngroup, double* part, hipStream_t s);
 hipError_t launch_hex_count(const Proj2DArgs& g, double dt, unsigned long long* out, hipStream_t s);
 hipError_t launch_tv_grad(const float* x, float* g, int nx, int ny, int batch, float eps, hipStream_t s);
+hipError_t launch_tv_descent(float* x, float* g, double* part, const double* step, int nx, int ny, int batch,
+                             float eps, int nsteps, hipStream_t s);
 hipError_t launch_resid(const float* b, const float* ax, const float* w, float* r, size_t n, size_t per, hipStream_t s);
 hipError_t launch_update(float* x, const float* c, const float* v, float lam, int positivity, size_t n, size_t per,
                          hipStream_t s);
@@ -1211,6 +1213,15 @@ int sgdct_axpy_batch(float* x, const float* g, const float* a, long long n, long
   if (!x || !g || !a || n <= 0 || per <= 0 || n % per) return fail(SGDCT_EINVAL, "bad axpy_batch arguments");
   hipError_t e = launch_axpy_batch(x, g, a, (size_t)n, (size_t)per, (hipStream_t)stream);
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "axpy_batch launch");
+}
+
+int sgdct_tv_descent(float* x, float* g, double* work, const double* step, int nx, int ny, int batch, float eps,
+                     int nsteps, void* stream) {
+  if (!x || !g || !work || !step || nx <= 0 || ny <= 0 || batch <= 0 || !(eps > 0) || nsteps < 0)
+    return fail(SGDCT_EINVAL, "bad tv_descent arguments");
+  if (nsteps == 0) return SGDCT_OK;
+  hipError_t e = launch_tv_descent(x, g, work, step, nx, ny, batch, eps, nsteps, (hipStream_t)stream);
+  return e == hipSuccess ? SGDCT_OK : hip_fail(e, "tv_descent launch");
 }
 
 int sgdct_tv_gradient(const float* x, float* g, int nx, int ny, int batch, float eps, void* stream) {

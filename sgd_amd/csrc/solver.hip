@@ -9,7 +9,10 @@
 //   * resid_kernel     r = W (b - A x), W = 1 / (A 1) the SIRT row weights;
 //   * update_kernel    x <- max(x + lam * V * c, 0) (positivity optional),
 //                      V = 1 / (A^T 1) the SIRT column weights;
-//   * axpy_kernel      x <- x + a * g.
+//   * axpy_kernel      x <- x + a * g;
+//   * sumsq_kernel / tv_step_kernel   one normalised TV-descent step,
+//                      x <- x - step * g / ||g|| per image, with ||g|| summed
+//                      on the device (no host round trip between steps).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -78,6 +81,61 @@ __global__ void axpy_batch_kernel(float* __restrict__ x, const float* __restrict
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   x[k] = __builtin_fmaf(a[k / per], g[k], x[k]);
+}
+
+// Per-image sums of squares in FP64: block (k, b) adds the elements
+// k * 256 + t + j * kTvParts * 256 of image b, then reduces its 256 values
+// in a fixed tree; part[b * kTvParts + k] holds the result.
+constexpr int kTvParts = 64;
+
+__global__ __launch_bounds__(256) void sumsq_kernel(const float* __restrict__ g, double* __restrict__ part,
+                                                    size_t per) {
+  __shared__ double red[256];
+  const float* gb = g + (size_t)blockIdx.y * per;
+  double acc = 0.0;
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < per; k += (size_t)kTvParts * 256) {
+    const double v = gb[k];
+    acc = __builtin_fma(v, v, acc);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * kTvParts + blockIdx.x] = red[0];
+}
+
+// x[b] -= step[b] * g[b] / ||g[b]|| (nothing where ||g[b]|| = 0); every block
+// of image b adds its kTvParts partial sums in the same order, so all use the
+// same norm.  grid (blocks per image, batch).
+__global__ __launch_bounds__(256) void tv_step_kernel(float* __restrict__ x, const float* __restrict__ g,
+                                                      const double* __restrict__ part,
+                                                      const double* __restrict__ step, size_t per) {
+  __shared__ float coef;
+  const int b = blockIdx.y;
+  if (threadIdx.x == 0) {
+    double ss = 0.0;
+    for (int k = 0; k < kTvParts; ++k) ss += part[(size_t)b * kTvParts + k];
+    const double gn = sqrt(ss);
+    coef = gn > 0.0 ? (float)(-step[b] / gn) : 0.0f;
+  }
+  __syncthreads();
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= per) return;
+  const size_t i = (size_t)b * per + k;
+  x[i] = __builtin_fmaf(coef, g[i], x[i]);
+}
+
+hipError_t launch_tv_descent(float* x, float* g, double* part, const double* step, int nx, int ny, int batch,
+                             float eps, int nsteps, hipStream_t s) {
+  const size_t per = (size_t)nx * ny;
+  for (int it = 0; it < nsteps; ++it) {
+    tv_grad_kernel<<<dim3((nx + 15) / 16, (ny + 15) / 16, batch), dim3(256), 0, s>>>(x, g, nx, ny, eps);
+    sumsq_kernel<<<dim3(kTvParts, batch), dim3(256), 0, s>>>(g, part, per);
+    tv_step_kernel<<<dim3((unsigned)((per + 255) / 256), batch), dim3(256), 0, s>>>(x, g, part, step, per);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_axpy_batch(float* x, const float* g, const float* a, size_t n, size_t per, hipStream_t s) {

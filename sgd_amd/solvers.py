@@ -98,6 +98,22 @@ def tv_gradient(x, eps=1e-8, out=None):
     return g
 
 
+def tv_descent(x, step, nsteps, eps=1e-8, g=None, work=None):
+    """In place on x (ny, nx) or (B, ny, nx): `nsteps` steps
+    x <- x - step * grad TV(x) / ||grad TV(x)|| per image, run on the device
+    without host synchronisation (`step`: one FP64 value per image, a device
+    tensor; 0 leaves the image unchanged, as does a zero gradient)."""
+    ny, nx = x.shape[-2:]
+    batch = x.numel() // (nx * ny)
+    g = torch.empty_like(x) if g is None else g
+    work = torch.empty(64 * batch, dtype=torch.float64, device=x.device) if work is None else work
+    step = step.to(torch.float64).contiguous()
+    assert x.is_contiguous() and g.shape == x.shape and step.numel() == batch and work.numel() >= 64 * batch
+    _lib.check(_lib.lib().sgdct_tv_descent(_vp(x), _vp(g), _vp(work), _vp(step), nx, ny, batch, float(eps),
+                                           int(nsteps), _stream(x)), "tv_descent")
+    return x
+
+
 class AsdPocsResult:
     def __init__(self, x, iters, residuals, seconds, fell_back):
         self.x, self.iters, self.residuals, self.seconds, self.fell_back = x, iters, residuals, seconds, fell_back
@@ -120,6 +136,8 @@ def asd_pocs(P, b, niter=200, tol=None, x0=None, beta=1.0, beta_red=0.995, ng=20
     hist = []
     fell_back = False
     g = torch.empty_like(x)
+    step = torch.zeros(1, dtype=torch.float64, device=b.device)
+    work = torch.empty(64, dtype=torch.float64, device=b.device)
     k = 0
     while k < niter:
         x_prev = x.clone()
@@ -140,12 +158,8 @@ def asd_pocs(P, b, niter=200, tol=None, x0=None, beta=1.0, beta_red=0.995, ng=20
         if dtvg is None:
             dtvg = alpha * dp
         x_tv0 = x.clone()
-        for _ in range(ng):
-            tv_gradient(x, tv_eps, out=g)
-            gn = float(torch.linalg.vector_norm(g.double()))
-            if gn == 0.0:
-                break
-            _lib.check(_lib.lib().sgdct_axpy(_vp(x), _vp(g), float(-dtvg / gn), x.numel(), _stream(x)), "axpy")
+        step.fill_(dtvg)
+        tv_descent(x, step, ng, tv_eps, g=g, work=work)
         dg = float(torch.linalg.vector_norm((x - x_tv0).double()))
         if dg > r_max * dp and res > eps_data:
             dtvg *= alpha_red
@@ -181,7 +195,6 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
     have alone: the projector gives each image of a batch the result it gets
     by itself, and all other steps are per image."""
     t0 = time.perf_counter()
-    L = _lib.lib()
     b = b.contiguous().float()
     nb = b.shape[0]
     w = weights or SirtWeights(P)
@@ -203,8 +216,6 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
     dtvg = torch.full((nb,), float("nan"), dtype=torch.float64, device=dev)
     tolw = None if tol is None else torch.as_tensor(tol, dtype=torch.float64, device=dev).expand(nb).clone()
     g = torch.empty_like(x)
-    coef = torch.empty(nb, device=dev)
-    per = x[0].numel()
     for k in range(niter):
         x_prev = x.clone()
         ax = sirt_step(P, w, x, bw, beta, positivity=True)
@@ -224,12 +235,7 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
         dp = _norms(x - x_prev)
         dtvg = torch.where(torch.isnan(dtvg), alpha * dp, dtvg)
         x_tv0 = x.clone()
-        for _ in range(ng):
-            tv_gradient(x, tv_eps, out=g)
-            gn = _norms(g)
-            c = torch.where(active & (gn > 0), -dtvg / gn.clamp_min(1e-300), torch.zeros_like(dtvg))
-            coef.copy_(c.float())
-            _lib.check(L.sgdct_axpy_batch(_vp(x), _vp(g), _vp(coef), x.numel(), per, _stream(x)), "axpy_batch")
+        tv_descent(x, torch.where(active, dtvg, torch.zeros_like(dtvg)), ng, tv_eps, g=g)
         dg = _norms(x - x_tv0)
         dtvg = torch.where((dg > r_max * dp) & (res > eps_data), dtvg * alpha_red, dtvg)
         beta *= beta_red
@@ -246,7 +252,6 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
                 tolw = tolw[keep]
                 fb = fb[keep].contiguous() if fb is not None else None
                 g = torch.empty_like(x)
-                coef = torch.empty(x.shape[0], device=dev)
     x_out[idx] = x
     fell[idx] = fellw
     torch.cuda.synchronize()

@@ -1,7 +1,8 @@
 """SIRT / ASD-POCS on the device (no reference counterpart: optimizer/asd_pocs.cu
 is empty).  Pins:
   * the TV-gradient kernel equals torch autograd of the same functional
-    (FP64 on the CPU);
+    (FP64 on the CPU), and the on-device normalised TV descent follows the
+    same steps taken in FP64 on the CPU;
   * SIRT weights: W * (A 1) = 1 on every ray that hits the image;
   * config 1 (B0, 256^2, 360 views, 512 bins, noise-free): 20 SIRT
     iterations decrease the data residual monotonically and move the image
@@ -28,6 +29,48 @@ def test_tv_gradient_matches_autograd(cuda):
     err = float((g - xr.grad).norm() / xr.grad.norm())
     print("tv grad rel err %.2e" % err)
     assert err <= 1e-5
+
+
+def test_tv_descent_matches_stepwise_fp64(cuda):
+    """sgdct_tv_descent (gradient, FP64 norm and normalised step on the
+    device) against the same steps in FP64 on the CPU, per image of a batch;
+    a zero step and a constant image (zero gradient) leave the image as it
+    is, runs repeat bit for bit, and an image in a batch gets its own result."""
+    import torch
+    from sgd_amd.solvers import tv, tv_descent
+    print("tv descent", flush=True)
+    rng = np.random.default_rng(1)
+    x0 = rng.random((3, 41, 29))
+    x0[2] = 0.7  # constant: zero gradient
+    steps = np.array([0.05, 0.02, 0.05])
+    eps, n = 1e-4, 12
+    xr = torch.tensor(x0, dtype=torch.float64)
+    for _ in range(n):
+        for b in range(3):
+            xb = xr[b].clone().requires_grad_(True)
+            tv(xb, eps=eps).backward()
+            gn = float(xb.grad.norm())
+            if gn > 0:
+                xr[b] -= steps[b] * xb.grad / gn
+    x = torch.tensor(x0, dtype=torch.float32, device=cuda)
+    step = torch.tensor(steps, dtype=torch.float64, device=cuda)
+    tv_descent(x, step, n, eps)
+    torch.cuda.synchronize()
+    xg = x.double().cpu()
+    for b in range(2):
+        err = float((xg[b] - xr[b]).norm() / (xr[b] - torch.tensor(x0[b])).norm())
+        print("image %d: error / change %.2e" % (b, err))
+        assert err <= 1e-4
+    assert torch.equal(xg[2], torch.tensor(x0[2], dtype=torch.float32).double())
+    x2 = torch.tensor(x0, dtype=torch.float32, device=cuda)
+    tv_descent(x2, step, n, eps)
+    assert torch.equal(x2, x)
+    x1 = torch.tensor(x0[1], dtype=torch.float32, device=cuda).contiguous()
+    tv_descent(x1, step[1:2], n, eps)
+    assert torch.equal(x1, x[1])
+    xz = torch.tensor(x0, dtype=torch.float32, device=cuda)
+    tv_descent(xz, torch.zeros(3, dtype=torch.float64, device=cuda), n, eps)
+    assert torch.equal(xz.cpu(), torch.tensor(x0, dtype=torch.float32))
 
 
 def _config1(basis="B0"):
