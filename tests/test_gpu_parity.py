@@ -45,6 +45,9 @@ CASES = {
     "drho": (63, 63, 2 * np.pi * np.arange(36) / 36, 96, 2.4, 2.4, 3.2, 1000.0, 500.0,
              5.0 * np.sin(np.arange(36))),
     "tight": (64, 64, 2 * np.pi * np.arange(45) / 45, 160, 0.5, 0.5, 0.4, 60.0, 30.07, None),
+    # 4 mm pixels on 0.25 mm bins: a 16-pixel tile spans ~512 bins, past the
+    # 128-bin LDS window, so A^T reads its bin records from global memory
+    "widewindow": (48, 48, 2 * np.pi * np.arange(30) / 30 + 0.05, 1024, 4.0, 4.0, 0.25, 1000.0, 500.3, None),
     "config1": (256, 256, 2 * np.pi * np.arange(360) / 360, 512, 1.0, 1.0, 1.0, 1000.0, 500.0, None),
 }
 
@@ -83,7 +86,7 @@ def test_sanity_library_loads_on_gpu(cuda):
 
 
 @pytest.mark.parametrize("basis", ["B1", "B0", "SF"])
-@pytest.mark.parametrize("case", ["tiny", "diagonals", "nonsquare", "offdetector", "drho", "tight"])
+@pytest.mark.parametrize("case", ["tiny", "diagonals", "nonsquare", "offdetector", "drho", "tight", "widewindow"])
 def test_forward_backward_parity(basis, case, cuda):
     if basis == "SF" and case == "drho":
         pytest.skip("SF_init takes no drho (layers/A_SF.cu:396)")
