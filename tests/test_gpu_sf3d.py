@@ -16,6 +16,9 @@ LSD, LSO = 200.0, 100.0
 CASES = {
     "small": dict(nx=24, ny=20, nz=18, np_=12, nu=48, nv=40, d=1.0, du=1.2, dv=1.1),
     "mid": dict(nx=64, ny=60, nz=48, np_=16, nu=128, nv=96, d=0.8, du=1.0, dv=1.0),
+    # the largest column and detector height the kernels stage (512 voxels,
+    # 512 rows)
+    "tall": dict(nx=8, ny=6, nz=512, np_=4, nu=24, nv=512, d=0.2, du=0.5, dv=0.4),
 }
 
 
