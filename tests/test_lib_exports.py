@@ -38,3 +38,16 @@ def test_split_config_rejects_bad_arguments_without_a_device():
     assert L.sgdct_split_config(None, 0, 0) == -1
     assert "split" in _lib.last_error()
     assert L.sgdct_split_config(None, 17, 0) == -1
+
+
+def test_tv_descent_rejects_bad_arguments_without_a_device():
+    """Null pointers, empty shapes, a non-positive eps or a negative step
+    count are SGDCT_EINVAL before any device work; zero steps do nothing."""
+    L = _lib.lib()
+    p = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks or returns first
+    assert L.sgdct_tv_descent(None, p, p, p, 8, 8, 1, 1e-8, 1, None) == -1
+    assert "tv_descent" in _lib.last_error()
+    assert L.sgdct_tv_descent(p, p, p, p, 0, 8, 1, 1e-8, 1, None) == -1
+    assert L.sgdct_tv_descent(p, p, p, p, 8, 8, 1, 0.0, 1, None) == -1
+    assert L.sgdct_tv_descent(p, p, p, p, 8, 8, 1, 1e-8, -1, None) == -1
+    assert L.sgdct_tv_descent(p, p, p, p, 8, 8, 1, 1e-8, 0, None) == 0
