@@ -334,7 +334,11 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
 #pragma unroll
   for (int k = 0; k < RCW; ++k) ycur[k] = ynext[k] = 0.0f;
   int n0, n1;
-  line_range(m_lo, n0, n1);
+  // m_lo is not clamped from above: a strip that misses the volume has
+  // m_lo > m_hi and loads nothing
+  n0 = 0;
+  n1 = -1;
+  if (m_lo <= m_hi) line_range(m_lo, n0, n1);
   if (n0 <= n1) load_col(m_lo, n0, ycur);
   for (int m = m_lo; m <= m_hi; ++m) {
     int q0 = 0, q1 = -1;  // next line's range

@@ -23,6 +23,9 @@ CASES = {
     # the largest column and detector height the kernels stage (512 voxels,
     # 512 rows)
     "tall": dict(nx=8, ny=6, nz=512, np_=4, nu=24, nv=512, d=0.2, du=0.5, dv=0.4),
+    # a detector three times wider than the volume: the strips of the outer
+    # bins miss it, so their walks have no line
+    "wide": dict(nx=16, ny=14, nz=10, np_=8, nu=96, nv=24, d=1.0, du=1.0, dv=1.0),
 }
 
 
