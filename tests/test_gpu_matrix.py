@@ -17,6 +17,7 @@ pytestmark = pytest.mark.gpu
 GEOMS = {
     "small": (8, 60, 24, 20, 1.0, 1.0, 1.2, 200.0, 100.0),
     "medium": (32, 171, 50, 50, 1.0, 1.0, 0.78125, 156.25, 78.125),
+    "tiny": (2, 5, 3, 2, 1.0, 1.0, 1.2, 200.0, 100.0),
 }
 
 
@@ -25,7 +26,8 @@ def _gpu_matrix(basis, geom, cap=None):
     np_, nu, nx, ny, dx, dy, du, lsd, lso = geom
     L = _lib.lib()
     f = L.bilinear_matrix_generation if basis == "B1" else L.pixel_matrix_generation
-    cap = cap if cap is not None else np_ * nu * nx * ny // 4
+    dense = np_ * nu * nx * ny
+    cap = cap if cap is not None else max(dense // 4, min(dense, 4096))  # small matrices are nearly dense
     buf = np.zeros((max(cap, 1), 3))
     n = f(np_, nu, nx, ny, dx, dy, du, lsd, lso, _lib.dptr(buf), cap)
     return n, buf[:max(n, 0)]
