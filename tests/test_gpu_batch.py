@@ -122,3 +122,24 @@ def test_autotuner_cache(cuda, tmp_path):
     assert _same(B.forward(img), B.forward_per_slice(img))
     assert _same(B.backward(sino), B.backward_per_slice(sino))
     print("tuned", {k: rec[k] for k in ("fwd_blocks_per_cu", "bwd_blocks_per_cu", "order", "fwd_ms", "bwd_ms")})
+
+
+def test_degenerate_slices_in_a_batch(cuda):
+    """Slices of one pixel, view or bin, and a few cells wide, mixed with
+    ordinary ones: the persistent launch equals the per-projector kernels."""
+    import torch
+    from sgd_amd.batch import SliceBatch, make_projectors, uneven_slices
+    print("batch with degenerate slices", flush=True)
+    specs = uneven_slices(6, seed=5, sizes=(32, 48), views=(20, 40), fov=96.0)
+    for basis, nx, ny, npv, nu in [("B1", 1, 1, 1, 1), ("SF", 3, 2, 2, 3), ("B0", 2, 5, 3, 1), ("B1", 5, 4, 7, 2)]:
+        specs.append(dict(basis=basis, nx=nx, ny=ny, np=npv, nu=nu, dx=2.0, dy=2.0, du=3.0, lsd=1000.0, lso=500.3,
+                          angles=2 * np.pi * np.arange(npv) / npv + 0.3))
+    B = SliceBatch(make_projectors(specs))
+    img, sino = _inputs(B, cuda, seed=4)
+    ref_s, ref_i = B.forward_per_slice(img), B.backward_per_slice(sino)
+    for fb, bb, order in [(8, 4, 0), (2, 1, 1)]:
+        B.set_config(fb, bb, order)
+        s, i = B.forward(img), B.backward(sino)
+        torch.cuda.synchronize()
+        assert _same(s, ref_s), (fb, bb, order)
+        assert _same(i, ref_i), (fb, bb, order)
