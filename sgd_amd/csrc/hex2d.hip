@@ -153,11 +153,16 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
     const int n0 = (int)lo, n1 = (int)hi;
     float ta = (float)((double)rec.ka * (sa * (n0 + sh) + ma * il + ca));
     float tb = (float)((double)rec.kb * (sb * (n0 + sh) + mb * il + cb));
-    const float* p = img + (size_t)il * g.nx + n0;
+    const float* p = img + (size_t)il * g.nx;
     float accl = 0.0f;
+    // the next point's value is requested one step ahead (the index clamped
+    // to the row's last point, so the read needs no branch), which overlaps
+    // its latency with the footprint evaluation
+    float xn = p[n0];
     for (int n = n0; n <= n1; ++n) {
-      accl = __builtin_fmaf(T::conv(rec, ta, tb), *p, accl);
-      ++p;
+      const float xc = xn;
+      xn = p[min(n + 1, n1)];
+      accl = __builtin_fmaf(T::conv(rec, ta, tb), xc, accl);
       ta += dta;
       tb += dtb;
     }

@@ -22,8 +22,16 @@ import torch
 
 
 def _time(fn, reps):
-    fn()
-    torch.cuda.synchronize()
+    # warm up for at least 20 calls and 50 ms: after the host-side set-up of a
+    # projector the GPU has been idle and its clocks take tens of
+    # milliseconds to come back (the first 30 B1 A calls on a new handle
+    # average ~10 % longer than the same calls a moment later)
+    t0 = time.perf_counter()
+    n = 0
+    while n < 20 or time.perf_counter() - t0 < 0.05:
+        fn()
+        torch.cuda.synchronize()
+        n += 1
     s = torch.cuda.current_stream()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record(s)
