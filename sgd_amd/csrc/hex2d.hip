@@ -137,11 +137,16 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
     il_lo = (int)fmax(floor(lo) - 1.0, 0.0);
     il_hi = (int)fmin(ceil(hi) + 1.0, (double)(g.ny - 1));
   }
+  // terms linear in the row index, carried from row to row in FP64
+  double q_il = mq * il_lo + cq, r_il = mr * il_lo + cr;
+  const double ka = rec.ka, kb = rec.kb;
+  double ta_il = ka * (ma * il_lo + ca), tb_il = kb * (mb * il_lo + cb);
+  const double dta_il = ka * ma, dtb_il = kb * mb, kas = ka * sa, kbs = kb * sb;
   double acc = 0.0;
-  for (int il = il_lo; il <= il_hi; ++il) {
+  for (int il = il_lo; il <= il_hi; ++il, q_il += mq, r_il += mr, ta_il += dta_il, tb_il += dtb_il) {
     const double sh = (il & 1) ? 0.5 : 0.0;
-    const double r1 = -rq - mq * il - cq - sq * sh;  // need sq * ic >= r1
-    const double r2 = rr - mr * il - cr - sr * sh;   // need sr * ic <  r2
+    const double r1 = -rq - q_il - sq * sh;  // need sq * ic >= r1
+    const double r2 = rr - r_il - sr * sh;   // need sr * ic <  r2
     double lo = 0.0, hi = (double)(g.nx - 1 - (il & 1));
     if (sq > 0.0) lo = fmax(lo, ceil(r1 * isq));
     else if (sq < 0.0) hi = fmin(hi, floor(r1 * isq));
@@ -151,8 +156,8 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
     else if (r2 <= 0.0) continue;
     if (lo > hi) continue;
     const int n0 = (int)lo, n1 = (int)hi;
-    float ta = (float)((double)rec.ka * (sa * (n0 + sh) + ma * il + ca));
-    float tb = (float)((double)rec.kb * (sb * (n0 + sh) + mb * il + cb));
+    float ta = (float)__builtin_fma(kas, n0 + sh, ta_il);
+    float tb = (float)__builtin_fma(kbs, n0 + sh, tb_il);
     const float* p = img + (size_t)il * g.nx;
     float accl = 0.0f;
     // the next point's value is requested one step ahead (the index clamped
