@@ -99,8 +99,8 @@ def cpu_baseline(img, sino, ang, basis, nviews, min_seconds=10.0):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(views would shard across GPUs with one all-reduce of A^T; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path "
+              "(views would shard across GPUs with one all-reduce of A^T)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/sgdct_oracle.c).
 
 Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it as the checker; the product (sgd_amd) never does.
+cpu_baseline leg use it to check results; the product (sgd_amd) never does.
 
 Two builds of the same source are exposed:
   Oracle("ref")    FP32 footprint arithmetic as in layers/A_*.cu, FP64 sums

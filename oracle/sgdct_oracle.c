@@ -3,8 +3,8 @@
  * projectors (B0, B1, SF, H0, Hl) and an FP64 strip-integral oracle.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load this library, and only as the checker or
- * as the timed CPU baseline.  The product (sgd_amd/) never links or calls it.
+ * cpu_baseline leg of bench.py load this library, and only to check results
+ * or as the timed CPU baseline.  The product (sgd_amd/) never links or calls it.
  *
  * The file is compiled twice (see oracle/Makefile):
  *   -DORACLE_FP=float   "reference mode": the footprint arithmetic is FP32

@@ -126,7 +126,7 @@ def asd_pocs(P, b, niter=200, tol=None, x0=None, beta=1.0, beta_red=0.995, ng=20
     or when ||A x - b|| / ||b|| <= tol.  If `fallback` is an image and the
     relative residual rises above the previous iteration's during the first
     iterations from x0, the run restarts from `fallback` (the warm start is
-    abandoned) -- the plain method then remains the judge."""
+    abandoned) -- the plain method then decides when to stop."""
     t0 = time.perf_counter()
     b = b.contiguous().float()
     w = weights or SirtWeights(P)

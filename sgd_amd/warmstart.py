@@ -13,7 +13,7 @@ b = -ln(max(counts, 1) / N0) / mu.
   use       x0 = UNet(SIRT(5)) seeds ASD-POCS, which runs until the relative
             data residual ||A x - b|| / ||b|| reaches the tolerance and
             restarts from the SIRT(5) image if its residual rises (the plain
-            method stays the judge: the stopping rule is the same).
+            method still decides when to stop: the stopping rule is the same).
 
 Tolerance.  The Poisson noise of this workload is 0.7 % of ||b|| (relative
 ||b - b_clean|| / ||b|| of 0.0068..0.009 on seeded phantoms), and with 90 x
