@@ -80,9 +80,9 @@ def batch(reps=3, tune_cache=None):
 
 
 def fan(reps=10):
-    """Every 2-D basis at its workload: B0 at config 1 (256^2, 1 mm, 360 views,
-    512 bins of 1 mm), B0 / B1 / SF at config 2 (512^2, 0.6 mm, 720 views, 768
-    bins of 0.8 mm), H0 / Hl on a 296 x 256 hexagonal lattice (dt 0.5 mm,
+    """Every 2-D basis at its workload: B0 at 256^2 (1 mm) with 360 views and
+    512 bins of 1 mm, B0 / B1 / SF at 512^2 (0.6 mm) with 720 views and 768
+    bins of 0.8 mm, H0 / Hl on a 296 x 256 hexagonal lattice (dt 0.5 mm,
     1024 views, 1024 bins of 0.6 mm) for one image and a batch of 64; lso 500,
     lsd 1000 mm; uniform random inputs."""
     from .torch_ops import Projector
@@ -114,7 +114,7 @@ def fan(reps=10):
 
 
 def asd(niter=50):
-    """Plain ASD-POCS at config 2: B1, 512^2 phantom (0.6 mm), 720 views,
+    """Plain ASD-POCS on B1 data: 512^2 phantom (0.6 mm), 720 views,
     768 bins (0.8 mm), Poisson noise at 1e5 photons on p = 0.1 A x, 50
     iterations with TV step ratio alpha = 0.2 (Sidky & Pan's controls); SIRT
     beside it at the same iteration count."""

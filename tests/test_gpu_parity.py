@@ -16,7 +16,7 @@ degrees (and with "round" magnifications other corners land exactly on other
 edges); there the reference's B0 corner rule floor(u) (layers/A_B0.cu:194-195)
 is discontinuous, its footprint reaches past the corner range, and any two
 implementations -- the reference's own FP32 build included -- may keep or drop
-a bin with a nonzero value.  The real configurations (config1 here) are tested
+a bin with a nonzero value.  The real workloads (fan256 here) are tested
 at their exact numbers, where such ties weigh ~4e-7.
 """
 import numpy as np
@@ -48,7 +48,7 @@ CASES = {
     # 4 mm pixels on 0.25 mm bins: a 16-pixel tile spans ~512 bins, past the
     # 128-bin LDS window, so A^T reads its bin records from global memory
     "widewindow": (48, 48, 2 * np.pi * np.arange(30) / 30 + 0.05, 1024, 4.0, 4.0, 0.25, 1000.0, 500.3, None),
-    "config1": (256, 256, 2 * np.pi * np.arange(360) / 360, 512, 1.0, 1.0, 1.0, 1000.0, 500.0, None),
+    "fan256": (256, 256, 2 * np.pi * np.arange(360) / 360, 512, 1.0, 1.0, 1.0, 1000.0, 500.0, None),
 }
 
 
@@ -123,16 +123,16 @@ def test_adjoint(basis, cuda):
 
 
 @pytest.mark.parametrize("basis", ["B0", "B1"])
-def test_config1_full(basis, cuda):
-    P, ge, gr = make(basis, "config1", cuda)
+def test_fan256_full_size(basis, cuda):
+    P, ge, gr = make(basis, "fan256", cuda)
     img = phantom(0, P.nx, P.ny, ge.dx, ge.dy)
     ya = gpu_fwd(P, img)
     ye = ge.forward(basis, img)
-    print("config1 %s A gpu-exact %.2e" % (basis, rel(ya, ye)))
+    print("fan256 %s A gpu-exact %.2e" % (basis, rel(ya, ye)))
     assert rel(ya, ye) <= TOL_EXACT
     xa = gpu_bwd(P, ye)
     xe = ge.backward(basis, ye)
-    print("config1 %s A^T gpu-exact %.2e" % (basis, rel(xa, xe)))
+    print("fan256 %s A^T gpu-exact %.2e" % (basis, rel(xa, xe)))
     assert rel(xa, xe) <= TOL_EXACT
 
 
@@ -142,7 +142,7 @@ def test_split_launches_agree(basis, cuda):
     partial sums added in a fixed order) give the unsplit result to one FP32
     rounding, and each split is deterministic."""
     import torch
-    P, ge, gr = make(basis, "config1", cuda)
+    P, ge, gr = make(basis, "fan256", cuda)
     g = torch.Generator(device="cuda").manual_seed(3)
     x = torch.rand(P.img_shape, device="cuda", generator=g)
     y = torch.rand(P.sino_shape, device="cuda", generator=g)

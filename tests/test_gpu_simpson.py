@@ -23,7 +23,7 @@ NX, NP, NU, DX, DU, LSD, LSO = 512, 720, 768, 0.6, 0.8, 1000.0, 500.0
 
 
 @pytest.mark.parametrize("basis", ["B1", "SF", "B0"])
-def test_config2_sampled_views_vs_simpson(cuda, oracle_exact, basis):
+def test_512px_720views_sampled_vs_simpson(cuda, oracle_exact, basis):
     import torch
     from pyoracle import Geometry
     from sgd_amd.torch_ops import Projector

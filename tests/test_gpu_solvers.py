@@ -4,7 +4,7 @@ is empty).  Pins:
     (FP64 on the CPU), and the on-device normalised TV descent follows the
     same steps taken in FP64 on the CPU;
   * SIRT weights: W * (A 1) = 1 on every ray that hits the image;
-  * config 1 (B0, 256^2, 360 views, 512 bins, noise-free): 20 SIRT
+  * B0 at 256^2 with 360 views and 512 bins, noise-free: 20 SIRT
     iterations decrease the data residual monotonically and move the image
     towards the phantom;
   * ASD-POCS lowers TV relative to SIRT at equal iterations on noisy data,
@@ -73,7 +73,7 @@ def test_tv_descent_matches_stepwise_fp64(cuda):
     assert torch.equal(xz.cpu(), torch.tensor(x0, dtype=torch.float32))
 
 
-def _config1(basis="B0"):
+def _fan256(basis="B0"):
     from sgd_amd.phantom import phantom
     from sgd_amd.torch_ops import Projector
     ang = 2 * np.pi * np.arange(360) / 360
@@ -81,11 +81,11 @@ def _config1(basis="B0"):
     return P, phantom(0, 256, 256, 1.0, 1.0)
 
 
-def test_sirt_config1(cuda):
+def test_sirt_256px_360views(cuda):
     import torch
     from sgd_amd.solvers import SirtWeights, sirt
-    print("sirt config1", flush=True)
-    P, img = _config1()
+    print("sirt 256^2 / 360 views", flush=True)
+    P, img = _fan256()
     x_true = torch.tensor(img, dtype=torch.float32, device=cuda)
     b = P.forward(x_true)
     w = SirtWeights(P)
