@@ -51,3 +51,27 @@ def test_tv_descent_rejects_bad_arguments_without_a_device():
     assert L.sgdct_tv_descent(p, p, p, p, 8, 8, 1, 0.0, 1, None) == -1
     assert L.sgdct_tv_descent(p, p, p, p, 8, 8, 1, 1e-8, -1, None) == -1
     assert L.sgdct_tv_descent(p, p, p, p, 8, 8, 1, 1e-8, 0, None) == 0
+
+
+def test_init_and_calls_reject_bad_arguments_without_a_device():
+    """Invalid geometry, sizes past the staged maxima (512 voxels per column,
+    512 detector rows) and null handles are errors -- a NULL handle or a
+    negative code, with a message -- before any device work."""
+    L = _lib.lib()
+    ang = (ctypes.c_double * 4)(0.0, 0.5, 1.0, 1.5)
+    assert L.b1_init(0, 8, 4, 16, 1.0, 1.0, 1.0, 1000.0, 500.0, ang, None, None) is None
+    assert "geometry" in _lib.last_error()
+    assert L.b0_init(8, 8, 4, 16, 0.0, 1.0, 1.0, 1000.0, 500.0, ang, None, None) is None
+    assert L.b1_init(8, 8, 4, 16, 1.0, 1.0, 1.0, 1000.0, 500.0, None, None, None) is None
+    assert L.SF_init(8, 8, 4, 0, 1.0, 1.0, 1.0, 1000.0, 500.0, ang) is None
+    assert L.h0_init(9, 8, 4, 16, 0.0, 1.0, 1000.0, 500.0, ang, None, None) is None
+    assert "dt" in _lib.last_error()
+    cone = (1.0, 1.0, 1.0, 1.0, 1.0, 1000.0, 500.0)
+    assert L.b1_3d_init(8, 8, 0, 4, 16, 16, *cone, ang) is None
+    assert L.b1_3d_init(8, 8, 513, 4, 16, 16, *cone, ang) is None
+    assert L.sf3d_init(8, 8, 16, 4, 16, 513, *cone, ang, 1) is None
+    assert L.b1_forward_projection(None, None, None) < 0
+    assert "null" in _lib.last_error()
+    assert L.h0_backward_projection(None, None, None) < 0
+    assert L.sgdct_forward_dev(None, None, None, 1, None) < 0
+    assert L.sgdct_backward_dev(None, None, None, 1, None) < 0
