@@ -134,13 +134,12 @@ int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int bat
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
 
 /* Fan-beam launch shape (no reference counterpart).  A splits every bin's
- * pixel walk into `fwd_split` line segments (B0 / B1 / SF; hexagonal handles
- * take 0 or 1) and A^T the views into `bwd_split` groups (every fan-beam
- * basis), each writing FP64 partial sums that a second kernel adds
+ * pixel walk into `fwd_split` line segments (lattice rows on hexagonal
+ * handles) and A^T the views into `bwd_split` groups, each writing FP64 partial sums that a second kernel adds
  * in a fixed order (results are deterministic, independent of the batch size,
  * and agree across splits to FP64 rounding).  0 = automatic: chosen at
  * creation from one image's launch size (a launch of few waves splits, up to
- * 16 ways); 1..16 = fixed.  The partial-sum workspace (2 images per launch,
+ * 16 ways; hexagonal A stays unsplit); 1..16 = fixed.  The partial-sum workspace (2 images per launch,
  * or up to 16 while it stays within 64 MiB; larger batches take several
  * launches) is allocated at creation and by this call, never inside a
  * projection.  Returns SGDCT_OK, SGDCT_EINVAL or SGDCT_ENOMEM. */

@@ -29,7 +29,7 @@ hipError_t launch_sf_backward(const Proj2DArgs& g, const float* sino, float* img
 void hex_bin_constants(int basis, const ViewGeo& v, int nu, double dt, void* out);
 size_t hex_bin_size(int basis);
 hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const float* img, float* sino, int batch,
-                              hipStream_t s);
+                              int nseg, double* part, hipStream_t s);
 hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const float* sino, float* img, int batch,
                                int ngroup, double* part, hipStream_t s);
 hipError_t launch_hex_count(const Proj2DArgs& g, double dt, unsigned long long* out, hipStream_t s);
@@ -258,10 +258,8 @@ void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, vo
 }
 
 bool is_hex(int basis) { return basis == BASIS_H0 || basis == BASIS_HL; }
-// A's walk splits into line segments for the square-lattice bases; A^T's
-// views split into groups for every fan-beam basis
-bool splits_fwd(int basis) { return basis == BASIS_B0 || basis == BASIS_B1 || basis == BASIS_SF; }
-bool splits_bwd(int basis) { return splits_fwd(basis) || is_hex(basis); }
+// fan-beam handles split both A (line or row segments) and A^T (view groups)
+bool splits(int basis) { return basis == BASIS_B0 || basis == BASIS_B1 || basis == BASIS_SF || is_hex(basis); }
 
 // ushift moves the bin edges by ushift bins: 0.5 gives cells centred on
 // integer detector coordinates (layers/SF_projector.cu:209-216).
@@ -334,8 +332,11 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   a.nchunk = (nu + 63) / 64;
   // 2 bins of slack for FP64 contraction differences between host and device
   if (basis == BASIS_B0 || basis == BASIS_B1) h->bwd_lds = max_tile_window(a, views.data()) <= kMaxWindow - 2;
-  if (splits_bwd(basis)) {
-    if (splits_fwd(basis)) h->fwd_auto = auto_fwd_split(h);
+  if (splits(basis)) {
+    // hexagonal A runs unsplit unless asked: at 296 x 256 points, 1024 views
+    // and 1024 bins, 2..16 row segments took one image from 1.11 to 1.20..1.40
+    // ms (H0) and a batch of 64 from 59 to 64..84 ms
+    h->fwd_auto = is_hex(basis) ? 1 : auto_fwd_split(h);
     h->bwd_auto = auto_bwd_split(h);
     if (ensure_part(h, part_images(h) * part_need(h)) != SGDCT_OK) {
       delete h;
@@ -490,6 +491,7 @@ int part_images(const sgdct_projector* h) {
 hipError_t launch_fwd2d(const sgdct_projector* h, const float* img, float* sino, int batch, int nseg, double* part,
                         hipStream_t s) {
   if (h->basis == BASIS_SF) return launch_sf_forward(h->args, img, sino, batch, nseg, part, s);
+  if (is_hex(h->basis)) return launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, nseg, part, s);
   return launch_forward(h->basis, h->args, img, sino, batch, nseg, part, s);
 }
 
@@ -503,9 +505,7 @@ hipError_t launch_bwd2d(const sgdct_projector* h, const float* sino, float* img,
 int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
   if (is_cone(h->basis)) return cone_forward(h, img, sino, batch, s);
   hipError_t e = hipSuccess;
-  if (is_hex(h->basis)) {
-    e = launch_hex_forward(h->basis, h->args, h->dt, img, sino, batch, s);
-  } else {
+  {
     const int nseg = h->fwd_split ? h->fwd_split : h->fwd_auto;
     const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)nseg * ns;
     if (nseg == 1) {
@@ -1062,9 +1062,7 @@ int sgdct_batch_graph_launch(sgdct_batch* b, int direction, const float* in, flo
 int sgdct_split_config(sgdct_projector* h, int fwd_split, int bwd_split) {
   if (!h || fwd_split < 0 || fwd_split > kMaxSplit || bwd_split < 0 || bwd_split > kMaxSplit)
     return fail(SGDCT_EINVAL, "split_config: splits must be 0 (automatic) or 1..16");
-  if (!splits_bwd(h->basis)) return fail(SGDCT_EINVAL, "split_config: fan-beam handles only");
-  if (!splits_fwd(h->basis) && fwd_split > 1)
-    return fail(SGDCT_EINVAL, "split_config: hexagonal handles split A^T only");
+  if (!splits(h->basis)) return fail(SGDCT_EINVAL, "split_config: fan-beam handles only");
   const int fs = h->fwd_split, bs = h->bwd_split;
   h->fwd_split = fwd_split;
   h->bwd_split = bwd_split;

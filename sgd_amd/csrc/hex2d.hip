@@ -96,9 +96,13 @@ struct HexArgs {
 };
 
 // A: bin-driven gather over lattice rows.
+// grid (waves / 4, batch, nseg).  With nseg > 1, segment k = blockIdx.z takes
+// rows [k L, k L + L) (L = ceil(nl / nseg)) of its strip and writes its FP64
+// sum, already scaled, to part[(k * batch + image) * np * nu + v * nu + u];
+// sum_parts_kernel adds the segments in order.
 template <int HB>
 __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* __restrict__ img,
-                                                      float* __restrict__ sino) {
+                                                      float* __restrict__ sino, double* __restrict__ part) {
   using T = HexTraits<HB>;
   const Proj2DArgs& g = ha.g;
   const int lane = threadIdx.x & 63;
@@ -107,8 +111,8 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
   if (v >= g.np) return;
   const int u = (wave - v * g.nchunk) * 64 + lane;
   if (u >= g.nu) return;
+  const int seg = blockIdx.z, nseg = gridDim.z;
   img += (size_t)blockIdx.y * g.nx * g.ny;
-  sino += (size_t)blockIdx.y * g.np * g.nu;
   const ViewGeo vg = g.views[v];
   const typename T::Bin rec = reinterpret_cast<const typename T::Bin*>(g.bins)[(size_t)v * g.nu + u];
 
@@ -136,6 +140,11 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
     const double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
     il_lo = (int)fmax(floor(lo) - 1.0, 0.0);
     il_hi = (int)fmin(ceil(hi) + 1.0, (double)(g.ny - 1));
+  }
+  if (nseg > 1) {
+    const int len = (g.ny + nseg - 1) / nseg;
+    il_lo = max(il_lo, seg * len);
+    il_hi = min(il_hi, seg * len + len - 1);
   }
   // terms linear in the row index, carried from row to row in FP64
   double q_il = mq * il_lo + cq, r_il = mr * il_lo + cr;
@@ -173,7 +182,9 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
     }
     acc += (double)accl;
   }
-  sino[(size_t)v * g.nu + u] = (float)(acc * g.dx * g.dy);
+  const size_t out = (size_t)v * g.nu + u, nsino = (size_t)g.np * g.nu;
+  if (nseg > 1) part[((size_t)seg * gridDim.y + blockIdx.y) * nsino + out] = acc * g.dx * g.dy;
+  else sino[blockIdx.y * nsino + out] = (float)(acc * g.dx * g.dy);
 }
 
 // A^T: lattice-point-driven gather, 16 x 16 point tiles, LDS bin windows.
@@ -367,12 +378,15 @@ void hex_bin_constants(int basis, const ViewGeo& v, int nu, double dt, void* out
 
 size_t hex_bin_size(int basis) { return basis == BASIS_H0 ? sizeof(H0Bin) : sizeof(HlBin); }
 
+// nseg > 1 needs part with nseg * batch * np * nu doubles.
 hipError_t launch_hex_forward(int basis, const Proj2DArgs& g, double dt, const float* img, float* sino, int batch,
-                              hipStream_t s) {
+                              int nseg, double* part, hipStream_t s) {
+  if (nseg < 1 || (nseg > 1 && !part)) return hipErrorInvalidValue;
   HexArgs ha{g, dt};
-  dim3 grid((g.np * g.nchunk + 3) / 4, batch), block(256);
-  if (basis == BASIS_H0) hex_fwd_kernel<BASIS_H0><<<grid, block, 0, s>>>(ha, img, sino);
-  else hex_fwd_kernel<BASIS_HL><<<grid, block, 0, s>>>(ha, img, sino);
+  dim3 grid((g.np * g.nchunk + 3) / 4, batch, nseg), block(256);
+  if (basis == BASIS_H0) hex_fwd_kernel<BASIS_H0><<<grid, block, 0, s>>>(ha, img, sino, part);
+  else hex_fwd_kernel<BASIS_HL><<<grid, block, 0, s>>>(ha, img, sino, part);
+  if (nseg > 1) return launch_sum_parts(part, sino, (size_t)batch * g.np * g.nu, nseg, s);
   return hipGetLastError();
 }
 

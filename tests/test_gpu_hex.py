@@ -137,24 +137,24 @@ def test_autograd_hexagonal_batch(basis):
 
 
 @pytest.mark.parametrize("basis", ["H0", "Hl"])
-def test_hex_split_backward_agrees(basis):
-    """A^T with 1..16 view groups (FP64 partial sums added in a fixed order)
-    gives the unsplit result to one FP32 rounding, each split is deterministic
-    and a batch gets each image's own result; A does not split on this
-    lattice."""
+def test_hex_split_agrees(basis):
+    """A with 1..16 row segments and A^T with 1..16 view groups (FP64 partial
+    sums added in a fixed order) give the unsplit results to one FP32
+    rounding, each split is deterministic and a batch gets each image's own
+    result."""
     import torch
-    from sgd_amd._lib import SgdctError
     P, ge, gr = make(basis, "small")
     g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.rand((3,) + tuple(P.img_shape), device="cuda", generator=g)
     y = torch.rand((3,) + tuple(P.sino_shape), device="cuda", generator=g)
-    P.set_split(0, 1)
-    x1 = P.backward(y)
-    for bs in (3, 7, 16, 0):
-        P.set_split(0, bs)
-        xs = P.backward(y)
+    P.set_split(1, 1)
+    y1, x1 = P.forward(x), P.backward(y)
+    for s in (2, 3, 7, 16, 0):
+        P.set_split(s, s)
+        ys, xs = P.forward(x), P.backward(y)
         torch.cuda.synchronize()
-        assert torch.allclose(xs, x1, rtol=2.4e-7, atol=1e-30), bs
-        assert torch.equal(P.backward(y), xs), bs
-        assert torch.equal(torch.stack([P.backward(y[i]) for i in range(3)]), xs), bs
-    with pytest.raises(SgdctError):
-        P.set_split(2, 0)
+        assert torch.allclose(ys, y1, rtol=2.4e-7, atol=1e-30), s
+        assert torch.allclose(xs, x1, rtol=2.4e-7, atol=1e-30), s
+        assert torch.equal(P.forward(x), ys) and torch.equal(P.backward(y), xs), s
+        assert torch.equal(torch.stack([P.forward(x[i]) for i in range(3)]), ys), s
+        assert torch.equal(torch.stack([P.backward(y[i]) for i in range(3)]), xs), s
