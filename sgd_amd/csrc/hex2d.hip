@@ -97,7 +97,8 @@ struct HexArgs {
 
 // A: bin-driven gather over lattice rows.
 // grid (waves / 4, batch, nseg).  With nseg > 1, segment k = blockIdx.z takes
-// rows [k L, k L + L) (L = ceil(nl / nseg)) of its strip and writes its FP64
+// lines [k L, k L + L) of its walk (rows or vertical lines, L = ceil(lines /
+// nseg)) and writes its FP64
 // sum, already scaled, to part[(k * batch + image) * np * nu + v * nu + u];
 // sum_parts_kernel adds the segments in order.
 template <int HB>
@@ -127,60 +128,120 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
   const double sq = -lay * g.dx, mq = lax * g.dy, cq = lax * vg.w0y - lay * vg.w0x;
   const double sr = -lby * g.dx, mr = lbx * g.dy, cr = lbx * vg.w0y - lby * vg.w0x;
   const double rq = hex_reach(-lay, lax, ha.dt), rr = hex_reach(-lby, lbx, ha.dt);
-  const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
-  const float dta = (float)(rec.ka * sa), dtb = (float)(rec.kb * sb);
 
-  // rows the strip can reach: between the crossings of its two boundary
-  // lines with the first and last column (ic' = 0 and nc), one row of margin
-  int il_lo = 0, il_hi = g.ny - 1;
-  if (mq != 0.0 && mr != 0.0) {
-    const double last = (double)g.nx;
-    const double c1 = (-rq - cq) / mq, c2 = (-rq - cq - sq * last) / mq;
-    const double c3 = (rr - cr) / mr, c4 = (rr - cr - sr * last) / mr;
-    const double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
-    il_lo = (int)fmax(floor(lo) - 1.0, 0.0);
-    il_hi = (int)fmin(ceil(hi) + 1.0, (double)(g.ny - 1));
-  }
-  if (nseg > 1) {
-    const int len = (g.ny + nseg - 1) / nseg;
-    il_lo = max(il_lo, seg * len);
-    il_hi = min(il_hi, seg * len + len - 1);
-  }
-  // terms linear in the row index, carried from row to row in FP64
-  double q_il = mq * il_lo + cq, r_il = mr * il_lo + cr;
   const double ka = rec.ka, kb = rec.kb;
-  double ta_il = ka * (ma * il_lo + ca), tb_il = kb * (mb * il_lo + cb);
-  const double dta_il = ka * ma, dtb_il = kb * mb, kas = ka * sa, kbs = kb * sb;
   double acc = 0.0;
-  for (int il = il_lo; il <= il_hi; ++il, q_il += mq, r_il += mr, ta_il += dta_il, tb_il += dtb_il) {
-    const double sh = (il & 1) ? 0.5 : 0.0;
-    const double r1 = -rq - q_il - sq * sh;  // need sq * ic >= r1
-    const double r2 = rr - r_il - sr * sh;   // need sr * ic <  r2
-    double lo = 0.0, hi = (double)(g.nx - 1 - (il & 1));
-    if (sq > 0.0) lo = fmax(lo, ceil(r1 * isq));
-    else if (sq < 0.0) hi = fmin(hi, floor(r1 * isq));
-    else if (r1 > 0.0) continue;
-    if (sr > 0.0) hi = fmin(hi, ceil(r2 * isr) - 1.0);
-    else if (sr < 0.0) lo = fmax(lo, floor(r2 * isr) + 1.0);
-    else if (r2 <= 0.0) continue;
-    if (lo > hi) continue;
-    const int n0 = (int)lo, n1 = (int)hi;
-    float ta = (float)__builtin_fma(kas, n0 + sh, ta_il);
-    float tb = (float)__builtin_fma(kbs, n0 + sh, tb_il);
-    const float* p = img + (size_t)il * g.nx;
-    float accl = 0.0f;
-    // the next point's value is requested one step ahead (the index clamped
-    // to the row's last point, so the read needs no branch), which overlaps
-    // its latency with the footprint evaluation
-    float xn = p[n0];
-    for (int n = n0; n <= n1; ++n) {
-      const float xc = xn;
-      xn = p[min(n + 1, n1)];
-      accl = __builtin_fmaf(T::conv(rec, ta, tb), xc, accl);
-      ta += dta;
-      tb += dtb;
+  // Walk lattice rows, or the vertical lines k = 2 ic + (il & 1) (x = k dx / 2,
+  // every other row) when the wave's middle ray gives a strip more points per
+  // vertical line (|ray_y| dx > 2 dy |ray_x|): steep strips cross every row
+  // with only a point or two in each.  The choice is uniform over the wave.
+  const int umid = min((wave - v * g.nchunk) * 64 + 32, g.nu - 1);
+  const double rmx = vg.e0x + (umid + 0.5) * vg.pux, rmy = vg.e0y + (umid + 0.5) * vg.puy;
+  if (fabs(rmy) * g.dx <= 2.0 * g.dy * fabs(rmx)) {
+    // rows the strip can reach: between the crossings of its two boundary
+    // lines with the first and last column (ic' = 0 and nc), one row of margin
+    int il_lo = 0, il_hi = g.ny - 1;
+    if (mq != 0.0 && mr != 0.0) {
+      const double last = (double)g.nx;
+      const double c1 = (-rq - cq) / mq, c2 = (-rq - cq - sq * last) / mq;
+      const double c3 = (rr - cr) / mr, c4 = (rr - cr - sr * last) / mr;
+      const double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
+      il_lo = (int)fmax(floor(lo) - 1.0, 0.0);
+      il_hi = (int)fmin(ceil(hi) + 1.0, (double)(g.ny - 1));
     }
-    acc += (double)accl;
+    if (nseg > 1) {
+      const int len = (g.ny + nseg - 1) / nseg;
+      il_lo = max(il_lo, seg * len);
+      il_hi = min(il_hi, seg * len + len - 1);
+    }
+    // terms linear in the row index, carried from row to row in FP64
+    double q_il = mq * il_lo + cq, r_il = mr * il_lo + cr;
+    double ta_il = ka * (ma * il_lo + ca), tb_il = kb * (mb * il_lo + cb);
+    const double dta_il = ka * ma, dtb_il = kb * mb, kas = ka * sa, kbs = kb * sb;
+    const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
+    const float dta = (float)kas, dtb = (float)kbs;
+    for (int il = il_lo; il <= il_hi; ++il, q_il += mq, r_il += mr, ta_il += dta_il, tb_il += dtb_il) {
+      const double sh = (il & 1) ? 0.5 : 0.0;
+      const double r1 = -rq - q_il - sq * sh;  // need sq * ic >= r1
+      const double r2 = rr - r_il - sr * sh;   // need sr * ic <  r2
+      double lo = 0.0, hi = (double)(g.nx - 1 - (il & 1));
+      if (sq > 0.0) lo = fmax(lo, ceil(r1 * isq));
+      else if (sq < 0.0) hi = fmin(hi, floor(r1 * isq));
+      else if (r1 > 0.0) continue;
+      if (sr > 0.0) hi = fmin(hi, ceil(r2 * isr) - 1.0);
+      else if (sr < 0.0) lo = fmax(lo, floor(r2 * isr) + 1.0);
+      else if (r2 <= 0.0) continue;
+      if (lo > hi) continue;
+      const int n0 = (int)lo, n1 = (int)hi;
+      float ta = (float)__builtin_fma(kas, n0 + sh, ta_il);
+      float tb = (float)__builtin_fma(kbs, n0 + sh, tb_il);
+      const float* p = img + (size_t)il * g.nx;
+      float accl = 0.0f;
+      // the next point's value is requested one step ahead (the index clamped
+      // to the row's last point, so the read needs no branch), which overlaps
+      // its latency with the footprint evaluation
+      float xn = p[n0];
+      for (int n = n0; n <= n1; ++n) {
+        const float xc = xn;
+        xn = p[min(n + 1, n1)];
+        accl = __builtin_fmaf(T::conv(rec, ta, tb), xc, accl);
+        ta += dta;
+        tb += dtb;
+      }
+      acc += (double)accl;
+    }
+  } else {
+    // vertical lines the strip can reach: between the crossings of its two
+    // boundary lines with the first and last row, one line of margin
+    const int kmax = 2 * g.nx - 2;
+    int k_lo = 0, k_hi = kmax;
+    if (sq != 0.0 && sr != 0.0) {
+      const double last = (double)(g.ny - 1);
+      const double c1 = (-rq - cq) / sq, c2 = (-rq - cq - mq * last) / sq;
+      const double c3 = (rr - cr) / sr, c4 = (rr - cr - mr * last) / sr;
+      const double lo = fmin(fmin(c1, c2), fmin(c3, c4)), hi = fmax(fmax(c1, c2), fmax(c3, c4));
+      k_lo = (int)fmax(floor(2.0 * lo) - 2.0, 0.0);
+      k_hi = (int)fmin(ceil(2.0 * hi) + 2.0, (double)kmax);
+    }
+    if (nseg > 1) {
+      const int len = (kmax + nseg) / nseg;
+      k_lo = max(k_lo, seg * len);
+      k_hi = min(k_hi, seg * len + len - 1);
+    }
+    const double imq = mq != 0.0 ? 1.0 / mq : 0.0, imr = mr != 0.0 ? 1.0 / mr : 0.0;
+    const double kam = ka * ma, kbm = kb * mb;
+    const float dta = (float)(2.0 * kam), dtb = (float)(2.0 * kbm);
+    for (int k = k_lo; k <= k_hi; ++k) {
+      const int par = k & 1;
+      const double xh = 0.5 * k;                   // ic' of the line
+      const double r1 = -rq - cq - sq * xh;        // need mq * il >= r1
+      const double r2 = rr - cr - sr * xh;         // need mr * il <  r2
+      double lo = (double)par, hi = (double)(g.ny - 1);
+      if (mq > 0.0) lo = fmax(lo, ceil(r1 * imq));
+      else if (mq < 0.0) hi = fmin(hi, floor(r1 * imq));
+      else if (r1 > 0.0) continue;
+      if (mr > 0.0) hi = fmin(hi, ceil(r2 * imr) - 1.0);
+      else if (mr < 0.0) lo = fmax(lo, floor(r2 * imr) + 1.0);
+      else if (r2 <= 0.0) continue;
+      if (lo > hi) continue;
+      int i0 = (int)lo, i1 = (int)hi;
+      i0 += (i0 ^ par) & 1;  // rows of the line's parity
+      i1 -= (i1 ^ par) & 1;
+      if (i0 > i1) continue;
+      float ta = (float)__builtin_fma(kam, (double)i0, ka * (sa * xh + ca));
+      float tb = (float)__builtin_fma(kbm, (double)i0, kb * (sb * xh + cb));
+      const float* p = img + (k >> 1);
+      float accl = 0.0f;
+      float xn = p[(size_t)i0 * g.nx];
+      for (int il = i0; il <= i1; il += 2) {
+        const float xc = xn;
+        xn = p[(size_t)min(il + 2, i1) * g.nx];
+        accl = __builtin_fmaf(T::conv(rec, ta, tb), xc, accl);
+        ta += dta;
+        tb += dtb;
+      }
+      acc += (double)accl;
+    }
   }
   const size_t out = (size_t)v * g.nu + u, nsino = (size_t)g.np * g.nu;
   if (nseg > 1) part[((size_t)seg * gridDim.y + blockIdx.y) * nsino + out] = acc * g.dx * g.dy;
