@@ -20,6 +20,10 @@ CASES = {
     "degenerate": (31, 27, np.array([0.0, np.pi / 6, np.pi / 3, np.pi / 2, np.pi, 4 * np.pi / 3, 0.4]), 80, 2.4,
                    3.0, 1000.0, 500.3),
     "offdetector": (41, 37, 2 * np.pi * np.arange(30) / 30, 20, 2.0, 3.0, 1000.0, 500.3),
+    # the 296 x 256 point / 1024-bin workload on 16 of its 1024 views, spread
+    # over both walks (rows, vertical lines) and the angles where they switch
+    "full": (296, 256, 2 * np.pi * np.array([0, 16, 43, 64, 85, 100, 128, 171, 200, 256, 300, 341, 427, 512,
+                                              683, 853]) / 1024, 1024, 0.5, 0.6, 1000.0, 500.0),
 }
 
 
@@ -51,7 +55,7 @@ def run(P, arr, fwd=True):
 
 
 @pytest.mark.parametrize("basis", ["H0", "Hl"])
-@pytest.mark.parametrize("case", ["small", "degenerate", "offdetector"])
+@pytest.mark.parametrize("case", ["small", "degenerate", "offdetector", "full"])
 def test_hex_parity(basis, case):
     P, ge, gr = make(basis, case)
     img = lattice_image(ge.nl, ge.nc, 1)
