@@ -134,8 +134,10 @@ int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int bat
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
 
 /* Fan-beam launch shape (no reference counterpart).  A splits every bin's
- * pixel walk into `fwd_split` line segments (lattice rows on hexagonal
- * handles) and A^T the views into `bwd_split` groups, each writing FP64 partial sums that a second kernel adds
+ * pixel walk into `fwd_split` line segments (on hexagonal handles: segments of
+ * the lattice rows, or of the vertical lattice lines a wave walks when its
+ * strips are steep) and A^T the views into `bwd_split` groups, each writing
+ * FP64 partial sums that a second kernel adds
  * in a fixed order (results are deterministic, independent of the batch size,
  * and agree across splits to FP64 rounding).  0 = automatic: chosen at
  * creation from one image's launch size (a launch of few waves splits, up to

@@ -335,7 +335,8 @@ sgdct_projector* create(int basis, int nx, int ny, int np, int nu, double dx, do
   if (splits(basis)) {
     // hexagonal A runs unsplit unless asked: at 296 x 256 points, 1024 views
     // and 1024 bins, 2..16 row segments took one image from 1.11 to 1.20..1.40
-    // ms (H0) and a batch of 64 from 59 to 64..84 ms
+    // ms (H0) and a batch of 64 from 59 to 64..84 ms (measured when A walked
+    // lattice rows only, before steep waves walked the vertical lines)
     h->fwd_auto = is_hex(basis) ? 1 : auto_fwd_split(h);
     h->bwd_auto = auto_bwd_split(h);
     if (ensure_part(h, part_images(h) * part_need(h)) != SGDCT_OK) {

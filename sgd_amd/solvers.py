@@ -53,10 +53,16 @@ def sirt_step(P, w, x, b, lam=1.0, positivity=True, ax=None):
     if ax is None:
         ax = P.forward(x)
     r = torch.empty_like(ax)
-    n, per = ax.numel(), ax[0].numel() if ax.dim() > 2 else ax.numel()
+    # W and V hold one sinogram / image; a batch repeats them every `per` values
+    n, per = ax.numel(), w.W.numel()
+    if n % per or b.numel() != n:
+        raise ValueError("sirt_step: sinogram of %d values does not hold whole copies of the %d-value weights"
+                         % (n, per))
     _lib.check(L.sgdct_sirt_residual(_vp(b), _vp(ax), _vp(w.W), _vp(r), n, per, _stream(x)), "sirt_residual")
     c = P.backward(r)
-    n, per = x.numel(), x[0].numel() if x.dim() > 2 else x.numel()
+    n, per = x.numel(), w.V.numel()
+    if n % per:
+        raise ValueError("sirt_step: image of %d values does not hold whole copies of the %d-value weights" % (n, per))
     _lib.check(L.sgdct_sirt_update(_vp(x), _vp(c), _vp(w.V), float(lam), int(bool(positivity)), n, per, _stream(x)),
                "sirt_update")
     return ax
@@ -115,6 +121,11 @@ def tv_descent(x, step, nsteps, eps=1e-8, g=None, work=None):
 
 
 class AsdPocsResult:
+    """`iters`: iterations run before the stopping check that met `tol` (the
+    data steps that produced `x`, counting an abandoned warm-start iteration),
+    or niter; `residuals`: the relative residual entering each iteration, NaN
+    for the iteration that switched to the fallback image."""
+
     def __init__(self, x, iters, residuals, seconds, fell_back):
         self.x, self.iters, self.residuals, self.seconds, self.fell_back = x, iters, residuals, seconds, fell_back
 
@@ -126,7 +137,9 @@ def asd_pocs(P, b, niter=200, tol=None, x0=None, beta=1.0, beta_red=0.995, ng=20
     or when ||A x - b|| / ||b|| <= tol.  If `fallback` is an image and the
     relative residual rises above the previous iteration's during the first
     iterations from x0, the run restarts from `fallback` (the warm start is
-    abandoned) -- the plain method then decides when to stop."""
+    abandoned) -- the plain method then decides when to stop.  That iteration
+    takes no TV step, counts as an iteration and decays beta like any other
+    (asd_pocs_batch follows the same rules)."""
     t0 = time.perf_counter()
     b = b.contiguous().float()
     w = weights or SirtWeights(P)
@@ -152,6 +165,7 @@ def asd_pocs(P, b, niter=200, tol=None, x0=None, beta=1.0, beta_red=0.995, ng=20
             fell_back = True
             dtvg = None
             hist.append(float("nan"))
+            beta *= beta_red
             k += 1
             continue
         dp = float(torch.linalg.vector_norm((x - x_prev).double()))
@@ -166,7 +180,7 @@ def asd_pocs(P, b, niter=200, tol=None, x0=None, beta=1.0, beta_red=0.995, ng=20
         beta *= beta_red
         k += 1
     torch.cuda.synchronize()
-    return AsdPocsResult(x, len([h for h in hist if h == h]), hist, time.perf_counter() - t0, fell_back)
+    return AsdPocsResult(x, k, hist, time.perf_counter() - t0, fell_back)
 
 
 def _norms(t):
@@ -225,7 +239,10 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
             done = active & (res <= tolw)
             iters[idx] = torch.where(done, torch.full_like(idx, k), iters[idx])
             active = active & ~done
+        rise = torch.zeros_like(active)
         if fb is not None and prev_res is not None:
+            # an image that switches to its fallback takes no TV step in this
+            # iteration and sets its TV step length at its next data step
             rise = active & ~fellw & (res > prev_res)
             x = torch.where(rise[:, None, None], fb, x)
             fellw = fellw | rise
@@ -233,9 +250,9 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
         prev_res = res
         x = torch.where(active[:, None, None], x, x_prev).contiguous()
         dp = _norms(x - x_prev)
-        dtvg = torch.where(torch.isnan(dtvg), alpha * dp, dtvg)
+        dtvg = torch.where(torch.isnan(dtvg) & ~rise, alpha * dp, dtvg)
         x_tv0 = x.clone()
-        tv_descent(x, torch.where(active, dtvg, torch.zeros_like(dtvg)), ng, tv_eps, g=g)
+        tv_descent(x, torch.where(active & ~rise, dtvg, torch.zeros_like(dtvg)), ng, tv_eps, g=g)
         dg = _norms(x - x_tv0)
         dtvg = torch.where((dg > r_max * dp) & (res > eps_data), dtvg * alpha_red, dtvg)
         beta *= beta_red

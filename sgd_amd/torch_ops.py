@@ -120,8 +120,9 @@ class Projector:
         return _lib.check(_lib.lib().sgdct_eval_count(self.handle), "eval_count")
 
     def set_split(self, fwd=0, bwd=0):
-        """Fan-beam launch shape: A line segments (lattice rows on hexagonal
-        handles), A^T view groups (0 = automatic)."""
+        """Fan-beam launch shape: A line segments (hexagonal handles: segments
+        of lattice rows or, for steep strips, of vertical lattice lines), A^T
+        view groups (0 = automatic)."""
         _lib.check(_lib.lib().sgdct_split_config(self.handle, int(fwd), int(bwd)), "split_config")
 
     def _check_in(self, t, tail):
