@@ -22,13 +22,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-NX = NY = 512
-NP_VIEWS = 720
-NU = 768
-DX = 0.6
-DU = 0.8
-LSD = 1000.0
-LSO = 500.0
+from sgd_amd.workloads import FAN512  # noqa: E402
+
+NX, NY = FAN512["nx"], FAN512["ny"]
+NP_VIEWS = FAN512["views"]
+NU = FAN512["nu"]
+DX = FAN512["dx"]
+DU = FAN512["du"]
+LSD = FAN512["lsd"]
+LSO = FAN512["lso"]
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per B1 call at this workload, summed over the call's kernels
 # (the projector kernel and the partial-sum kernel), from rocprofv3 --pmc
@@ -105,16 +107,13 @@ def main():
     import numpy as np
     import torch
 
-    from sgd_amd.phantom import phantom, poisson_noise
-    from sgd_amd.torch_ops import Projector
+    from sgd_amd import workloads
 
     dev = torch.device("cuda", 0)
-    ang = 2 * np.pi * np.arange(NP_VIEWS) / NP_VIEWS
-    P = Projector(args.basis, NX, NY, ang, NU, DX, DX, DU, LSD, LSO)
-    img_np = phantom(0, NX, NY, DX, DX)
+    ang = workloads.angles(NP_VIEWS)
+    P = workloads.fan512_projector(args.basis)
+    img_np, sino_np = workloads.fan512_inputs(P)
     x = torch.tensor(img_np, dtype=torch.float32, device=dev)
-    y0 = P.forward(x)
-    sino_np = poisson_noise(y0.double().cpu().numpy() * 0.1, 1e5, seed=1) * 10.0
     y = torch.tensor(sino_np, dtype=torch.float32, device=dev)
     yo = torch.empty_like(y)
     xo = torch.empty_like(x)

@@ -120,6 +120,15 @@ int sf3d_backward_projection(double *b, double *x, sgdct_projector *h);
 
 /* ---- additions ------------------------------------------------------------ */
 
+/* SF_init with per-view source offsets, as b0_init / b1_init take them
+ * (layers/A_B1.hh:24; utils/geo.cu:243-251: the source of view p sits at
+ * R(angle_p)(-lso + drho[p], 0, dz[p])), for fan slices rebinned from a
+ * helical scan; the reference's SF_init (A_SF.cu:396) has no offsets.  Either
+ * pointer may be NULL; dz does not change a 2-D fan projection. */
+sgdct_projector *sgdct_sf_init_offsets(int nx, int ny, int np, int nu, double dx, double dy, double du,
+                                       double lsd, double lso, const double *angles, const double *dz,
+                                       const double *drho);
+
 /* Release a handle from any *_init (the reference leaks them). */
 void sgdct_destroy(sgdct_projector *h);
 

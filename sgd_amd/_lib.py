@@ -46,6 +46,8 @@ def lib():
         f.restype = _h
     L.SF_init.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [_dp]
     L.SF_init.restype = _h
+    L.sgdct_sf_init_offsets.argtypes = init12
+    L.sgdct_sf_init_offsets.restype = _h
     for name in ("h0_init", "hl_init"):
         f = getattr(L, name)
         f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 4 + [_dp, _dp, _dp]

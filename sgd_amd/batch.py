@@ -36,15 +36,18 @@ def default_cache_path():
 
 
 def uneven_slices(nslice=256, seed=0, sizes=(128, 256, 384, 512, 768), views=(180, 1440),
-                  bases=("B0", "B1", "SF"), fov=256.0, lsd=1000.0, lso=500.0):
+                  bases=("B0", "B1", "SF"), fov=256.0, lsd=1000.0, lso=500.0, helical=False):
     """Specs of fan slices of uneven size: image edge uniform over `sizes`
     (field of view `fov` mm), view count uniform over [views[0], views[1]]
-    with a per-slice start angle (helically rebinned slices), 1.5x as many
-    bins as the image edge, detector pitch covering the field of view, basis
-    uniform over `bases`."""
+    with a per-slice start angle, 1.5x as many bins as the image edge,
+    detector pitch covering the field of view, basis uniform over `bases`.
+    With `helical`, every view also carries the source offsets of a slice
+    rebinned from a helical scan (sgd_amd.workloads.helical_offsets: axial dz
+    over one pitch, radial drho of a flying focal spot)."""
+    from .workloads import helical_offsets
     rng = np.random.default_rng(seed)
     specs = []
-    for _ in range(nslice):
+    for i in range(nslice):
         n = int(rng.choice(sizes))
         nv = int(rng.integers(views[0], views[1] + 1))
         nu = int(round(1.5 * n))
@@ -53,12 +56,14 @@ def uneven_slices(nslice=256, seed=0, sizes=(128, 256, 384, 512, 768), views=(18
         start = float(rng.uniform(0, 2 * np.pi))
         specs.append(dict(basis=str(rng.choice(bases)), nx=n, ny=n, np=nv, nu=nu, dx=fov / n, dy=fov / n, du=du,
                           lsd=lsd, lso=lso, angles=start + 2 * np.pi * np.arange(nv) / nv))
+        if helical:
+            specs[-1]["dz"], specs[-1]["drho"] = helical_offsets(nv, seed=1000003 * seed + i)
     return specs
 
 
 def make_projectors(specs):
     return [Projector(s["basis"], s["nx"], s["ny"], s["angles"], s["nu"], s["dx"], s["dy"], s["du"], s["lsd"],
-                      s["lso"]) for s in specs]
+                      s["lso"], drho=s.get("drho"), dz=s.get("dz")) for s in specs]
 
 
 class SliceBatch:

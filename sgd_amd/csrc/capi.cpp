@@ -503,40 +503,52 @@ hipError_t launch_bwd2d(const sgdct_projector* h, const float* sino, float* img,
   return launch_backward(h->basis, h->args, h->bwd_lds, sino, img, batch, ngroup, part, s);
 }
 
-int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
-  if (is_cone(h->basis)) return cone_forward(h, img, sino, batch, s);
+// Fan-beam A of `batch` images with each bin's walk in `nseg` line segments,
+// the FP64 partial sums in part[0, cap) (doubles): as many images per launch
+// as the buffer holds, so every image is computed the same way whatever the
+// batch size or the buffer.
+int fan_forward(sgdct_projector* h, const float* img, float* sino, int batch, int nseg, double* part, size_t cap,
+                hipStream_t s) {
   hipError_t e = hipSuccess;
-  {
-    const int nseg = h->fwd_split ? h->fwd_split : h->fwd_auto;
-    const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)nseg * ns;
-    if (nseg == 1) {
-      e = launch_fwd2d(h, img, sino, batch, 1, nullptr, s);
-    } else {
-      if (h->part_cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
-      const int bc = (int)std::min<size_t>(batch, h->part_cap / per);
-      for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
-        e = launch_fwd2d(h, img + b0 * ni, sino + b0 * ns, std::min(bc, batch - b0), nseg, h->d_part, s);
-    }
+  const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)nseg * ns;
+  if (nseg == 1) {
+    e = launch_fwd2d(h, img, sino, batch, 1, nullptr, s);
+  } else {
+    if (!part || cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
+    const int bc = (int)std::min<size_t>(batch, cap / per);
+    for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
+      e = launch_fwd2d(h, img + b0 * ni, sino + b0 * ns, std::min(bc, batch - b0), nseg, part, s);
   }
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "forward launch");
 }
 
-int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
-  if (is_cone(h->basis)) return cone_backward(h, sino, img, batch, s);
+// Fan-beam A^T with the views in `ngroup` groups, partial sums as above.
+int fan_backward(sgdct_projector* h, const float* sino, float* img, int batch, int ngroup, double* part, size_t cap,
+                 hipStream_t s) {
   hipError_t e = hipSuccess;
-  {
-    const int ng = h->bwd_split ? h->bwd_split : h->bwd_auto;
-    const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)ng * ni;
-    if (ng == 1) {
-      e = launch_bwd2d(h, sino, img, batch, 1, nullptr, s);
-    } else {
-      if (h->part_cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
-      const int bc = (int)std::min<size_t>(batch, h->part_cap / per);
-      for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
-        e = launch_bwd2d(h, sino + b0 * ns, img + b0 * ni, std::min(bc, batch - b0), ng, h->d_part, s);
-    }
+  const size_t ni = (size_t)h->nx * h->ny, ns = (size_t)h->np * h->nu, per = (size_t)ngroup * ni;
+  if (ngroup == 1) {
+    e = launch_bwd2d(h, sino, img, batch, 1, nullptr, s);
+  } else {
+    if (!part || cap < per) return fail(SGDCT_EDEVICE, "partial-sum workspace missing");
+    const int bc = (int)std::min<size_t>(batch, cap / per);
+    for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += bc)
+      e = launch_bwd2d(h, sino + b0 * ns, img + b0 * ni, std::min(bc, batch - b0), ngroup, part, s);
   }
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, "backward launch");
+}
+
+int fwd_segments(const sgdct_projector* h) { return h->fwd_split ? h->fwd_split : h->fwd_auto; }
+int bwd_groups(const sgdct_projector* h) { return h->bwd_split ? h->bwd_split : h->bwd_auto; }
+
+int run_forward(sgdct_projector* h, const float* img, float* sino, int batch, hipStream_t s) {
+  if (is_cone(h->basis)) return cone_forward(h, img, sino, batch, s);
+  return fan_forward(h, img, sino, batch, fwd_segments(h), h->d_part, h->part_cap, s);
+}
+
+int run_backward(sgdct_projector* h, const float* sino, float* img, int batch, hipStream_t s) {
+  if (is_cone(h->basis)) return cone_backward(h, sino, img, batch, s);
+  return fan_backward(h, sino, img, batch, bwd_groups(h), h->d_part, h->part_cap, s);
 }
 
 // Footprint evaluations of the reference's cone-beam forward
@@ -818,6 +830,12 @@ sgdct_projector* SF_init(int nx, int ny, int np, int nu, double dx, double dy, d
   return create(BASIS_SF, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, nullptr);
 }
 int SF_forward_projection(double* b, double* x, sgdct_projector* h) { return host_forward(b, x, h); }
+
+sgdct_projector* sgdct_sf_init_offsets(int nx, int ny, int np, int nu, double dx, double dy, double du, double lsd,
+                                       double lso, const double* angles, const double* dz, const double* drho) {
+  (void)dz;  // out-of-plane source offset: no effect on a 2-D fan (z = 0 slice)
+  return create(BASIS_SF, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, drho);
+}
 int SF_backward_projection(double* b, double* x, sgdct_projector* h) { return host_backward(b, x, h); }
 
 // layers/A_H0.hh:24 / A_H0.cu:509-643 and A_Hl.hh:24 / A_Hl.cu:521-657:

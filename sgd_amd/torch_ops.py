@@ -17,9 +17,11 @@ BASES = {"B0": "b0", "B1": "b1", "SF": "SF", "H0": "h0", "Hl": "hl"}
 
 
 class Projector:
-    def __init__(self, basis, nx, ny, angles, nu, dx, dy, du, lsd, lso, drho=None):
+    def __init__(self, basis, nx, ny, angles, nu, dx, dy, du, lsd, lso, drho=None, dz=None):
         """Square lattices (B0, B1, SF): nx x ny pixels of dx x dy.  For the
-        hexagonal bases use Projector.hexagonal."""
+        hexagonal bases use Projector.hexagonal.  `drho` / `dz`: per-view
+        radial / axial source offsets (layers/A_B1.hh:24; dz does not change a
+        2-D fan projection); SF takes them through sgdct_sf_init_offsets."""
         if basis not in BASES:
             raise ValueError("unknown basis %r" % (basis,))
         if not torch.cuda.is_available():
@@ -33,21 +35,32 @@ class Projector:
         L = _lib.lib()
         torch.cuda.init()
         pre = BASES[basis]
+
+        def per_view(a, name):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.size != self.np:
+                raise ValueError("%s needs one value per view (%d), got %d" % (name, self.np, a.size))
+            return a
+
+        dr, dzv = per_view(drho, "drho"), per_view(dz, "dz")
+        self._drho, self._dz = dr, dzv
+        p_dr = None if dr is None else _lib.dptr(dr)
+        p_dz = None if dzv is None else _lib.dptr(dzv)
         if basis in ("H0", "Hl"):
             # hexagonal: nx = nc columns, ny = nl rows, dx = dt (lattice constant)
-            dr = None if drho is None else np.ascontiguousarray(drho, dtype=np.float64)
-            self._drho = dr
             h = getattr(L, pre + "_init")(self.ny, self.nx, self.np, self.nu, dx, du, lsd, lso, _lib.dptr(ang),
-                                          None, None if dr is None else _lib.dptr(dr))
+                                          p_dz, p_dr)
         elif pre == "SF":
-            if drho is not None:
-                raise ValueError("SF_init takes no drho (layers/A_SF.cu:396)")
-            h = L.SF_init(self.nx, self.ny, self.np, self.nu, dx, dy, du, lsd, lso, _lib.dptr(ang))
+            if dr is None and dzv is None:
+                h = L.SF_init(self.nx, self.ny, self.np, self.nu, dx, dy, du, lsd, lso, _lib.dptr(ang))
+            else:
+                h = L.sgdct_sf_init_offsets(self.nx, self.ny, self.np, self.nu, dx, dy, du, lsd, lso,
+                                            _lib.dptr(ang), p_dz, p_dr)
         else:
-            dr = None if drho is None else np.ascontiguousarray(drho, dtype=np.float64)
-            self._drho = dr
             h = getattr(L, pre + "_init")(self.nx, self.ny, self.np, self.nu, dx, dy, du, lsd, lso,
-                                          _lib.dptr(ang), None, None if dr is None else _lib.dptr(dr))
+                                          _lib.dptr(ang), p_dz, p_dr)
         if not h:
             raise _lib.SgdctError("%s_init failed: %s" % (pre, _lib.last_error()))
         self.handle = h
@@ -56,13 +69,13 @@ class Projector:
         self._sino_shape = (self.np, self.nu)
 
     @classmethod
-    def hexagonal(cls, basis, nl, nc, angles, nu, dt, du, lsd, lso, drho=None):
+    def hexagonal(cls, basis, nl, nc, angles, nu, dt, du, lsd, lso, drho=None, dz=None):
         """H0 / Hl on nl rows x nc columns (images (nl, nc), [il][ic]); odd
         rows hold nc-1 points, their last column is ignored by A and zeroed
         by A^T."""
         if basis not in ("H0", "Hl"):
             raise ValueError("hexagonal bases are H0 and Hl")
-        return cls(basis, nc, nl, angles, nu, dt, dt, du, lsd, lso, drho=drho)
+        return cls(basis, nc, nl, angles, nu, dt, dt, du, lsd, lso, drho=drho, dz=dz)
 
     @classmethod
     def cone_beam(cls, nx, ny, nz, angles, nu, nv, dx, dy, dz, du, dv, lsd, lso, model="B1"):

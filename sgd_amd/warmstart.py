@@ -127,16 +127,19 @@ def train(net, inputs, targets, epochs=8, batch=8, lr=1e-3, seed=0, log=print):
     s = 0
     for ep in range(epochs):
         order = rng.permutation(n)
-        run = 0.0
+        run = torch.zeros((), dtype=torch.float64, device=inputs.device)
+        nstep = 0
         for k in range(0, n - batch + 1, batch):
             idx = torch.as_tensor(order[k:k + batch], device=inputs.device)
             lr_s = lr * 0.5 * (1 + np.cos(np.pi * s / steps))
             y = net.train_step(inputs[idx], targets[idx], lr=lr_s)
+            loss = ((y - targets[idx]).double() ** 2).mean()
+            run += loss
+            nstep += 1
             if s % 50 == 0:
-                run = float(((y - targets[idx]) ** 2).mean())
-                losses.append((s, run))
+                losses.append((s, float(loss)))
             s += 1
-        log("epoch %d  loss %.4e  %.1fs" % (ep, run, time.perf_counter() - t0))
+        log("epoch %d  mean loss %.4e  %.1fs" % (ep, float(run) / max(nstep, 1), time.perf_counter() - t0))
     return losses
 
 
