@@ -142,6 +142,26 @@ void sgdct_destroy(sgdct_projector *h);
 int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  sgdct_forward_dev (direction 0: in = images, out = sinograms)
+ * or sgdct_backward_dev (direction 1: in = sinograms, out = images) on a
+ * fan-beam handle (B0, B1, SF, H0, Hl), with the FP64 partial sums of split
+ * launches in a static device array of 32 MiB per GPU (allocated when the
+ * library's code is loaded on that GPU, held until the process ends) instead
+ * of the handle's workspace.  The call allocates nothing; its first call on a
+ * GPU looks up the array's address there.  The handle's device must be
+ * current.  A split chosen automatically is lowered, when one image's partial
+ * sums do not fit in 32 MiB, to the largest that fits (results agree with
+ * sgdct_*_dev to FP64 rounding of the partial sums; bit for bit at the same
+ * split); a split fixed by sgdct_split_config that does not fit is
+ * SGDCT_EINVAL.  Cone-beam handles are SGDCT_EUNSUP.  Every call on a GPU
+ * shares that GPU's array (each launch writes every partial sum it reads, so
+ * nothing needs clearing): calls that overlap on one GPU, from several
+ * streams or threads, race. */
+int sgdct_project_dev_noalloc(sgdct_projector *h, int direction, const float *in, float *out, int batch,
+                              void *stream);
+
 /* Fan-beam launch shape (no reference counterpart).  A splits every bin's
  * pixel walk into `fwd_split` line segments (on hexagonal handles: segments of
  * the lattice rows, or of the vertical lattice lines a wave walks when its

@@ -63,6 +63,8 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [_h, _fp, _fp, ctypes.c_int, _fp]
         f.restype = ctypes.c_int
+    L.sgdct_project_dev_noalloc.argtypes = [_h, ctypes.c_int, _fp, _fp, ctypes.c_int, _fp]
+    L.sgdct_project_dev_noalloc.restype = ctypes.c_int
     L.sgdct_split_config.argtypes = [_h, ctypes.c_int, ctypes.c_int]
     L.sgdct_split_config.restype = ctypes.c_int
     L.sgdct_shape.argtypes = [_h, ctypes.POINTER(ctypes.c_int)]

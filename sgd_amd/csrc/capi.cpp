@@ -7,6 +7,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -78,6 +79,14 @@ hipError_t launch_batch_backward(const BatchPlan& P, int blocks, const float* si
 
 using namespace sgdct;
 
+// Partial sums of sgdct_project_dev_noalloc (FP64, one array per GPU, alive
+// until the process ends).  Host code never passes `scratch` itself to a
+// kernel -- on the host it names a stand-in, not device memory -- but the
+// device address scratch_on() reads with hipGetSymbolAddress.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -90,6 +99,26 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? SGDCT_ENOMEM : SGDCT_EDEVICE,
               std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// once per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(SGDCT_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "scratch array address");
+    if (!p) return fail(SGDCT_EDEVICE, "scratch array address is null");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return SGDCT_OK;
 }
 
 // Source, detector centre and detector step of one view, utils/geo.cu:240-262:
@@ -1101,6 +1130,38 @@ int sgdct_forward_dev(sgdct_projector* h, const float* img, float* sino, int bat
 int sgdct_backward_dev(sgdct_projector* h, const float* sino, float* img, int batch, void* stream) {
   if (!h || !img || !sino || batch <= 0) return fail(SGDCT_EINVAL, "bad backward_dev arguments");
   return run_backward(h, sino, img, batch, (hipStream_t)stream);
+}
+
+int sgdct_project_dev_noalloc(sgdct_projector* h, int direction, const float* in, float* out, int batch,
+                              void* stream) {
+  if (!h || !in || !out || batch <= 0 || (direction != 0 && direction != 1))
+    return fail(SGDCT_EINVAL, "bad project_dev_noalloc arguments");
+  if (is_cone(h->basis))
+    return fail(SGDCT_EUNSUP, "project_dev_noalloc: cone-beam handles stage their views in per-handle "
+                              "workspace; use sgdct_forward_dev / sgdct_backward_dev");
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "get device");
+  if (dev != h->device) return fail(SGDCT_EINVAL, "project_dev_noalloc: the handle's device is not current");
+  double* part = nullptr;
+  if (int rc = scratch_on(dev, &part)) return rc;
+  // The handle's split, lowered (when it was chosen automatically) to the
+  // largest whose partial sums of one image fit in the scratch array; a split
+  // fixed by sgdct_split_config that does not fit is an error.
+  int n = direction == 0 ? fwd_segments(h) : bwd_groups(h);
+  const bool fixed = direction == 0 ? h->fwd_split != 0 : h->bwd_split != 0;
+  const size_t unit = direction == 0 ? (size_t)h->np * h->nu : (size_t)h->nx * h->ny;
+  if (n > 1 && (size_t)n * unit > kScratchDoubles) {
+    if (fixed)
+      return fail(SGDCT_EINVAL, "project_dev_noalloc: one image at the fixed split " + std::to_string(n) + " needs " +
+                                    std::to_string((size_t)n * unit * sizeof(double)) +
+                                    " bytes of partial sums, more than the " + std::to_string(kScratchBytes) +
+                                    "-byte scratch array (lower it with sgdct_split_config)");
+    n = (int)std::max<size_t>(1, kScratchDoubles / unit);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return direction == 0 ? fan_forward(h, in, out, batch, n, part, kScratchDoubles, s)
+                        : fan_backward(h, in, out, batch, n, part, kScratchDoubles, s);
 }
 
 int sgdct_shape(const sgdct_projector* h, int* out) {
