@@ -247,28 +247,33 @@ void bin_constants(int basis, const ViewGeo& v, int nu, double dx, double dy, vo
     const double th = std::atan2(std::fabs(cx), std::fabs(cy));
     const float vx = (float)std::fabs(dx * std::cos(th)), vy = (float)std::fabs(dy * std::sin(th));
     if (basis == BASIS_B1) {
-      // constants in the scaled offset t' = s t (footprint.hpp)
+      // constants in the offset t' = s t scaled to the support (footprint.hpp)
       B1Bin r{};
       const double a = std::max(vx, vy);
-      double b = std::min(vx, vy);
-      double sc;
+      const double b = std::min(vx, vy);
+      double sc, scale;
       if (b > 1e-7) {
-        sc = std::pow(24.0 * a * a * b * b, -0.25);  // c^(1/4)
-        const double bs = b * sc;
+        sc = 1.0 / (a + b);
+        const double as = a * sc, bs = b * sc;
+        const double h = 12.0 * as * as * bs * bs;  // 1 / (2 c')
+        r.h = (float)h;
+        r.b = (float)bs;
+        r.a = (float)as;
         r.nk2 = (float)(-12.0 * bs * bs);
-        r.h4 = (float)(0.5 - 2.0 * bs * bs * bs * bs);
+        r.hk4 = (float)(h - 2.0 * bs * bs * bs * bs);
+        scale = sc / (2.0 * h);  // s c'
       } else {  // 1tri1box limit, spline_conv.hh:393-405
-        b = 0.0;
         sc = 1.0 / a;
+        r.h = 0.5f;
+        r.b = 0.0f;
+        r.a = 1.0f;
         r.nk2 = -0.5f;
-        r.h4 = 0.5f;
+        r.hk4 = 0.5f;
+        scale = sc;
       }
       r.ka = (float)(ka * sc);
       r.kb = (float)(kb * sc);
-      r.a = (float)(a * sc);
-      r.b = (float)(b * sc);
-      r.apb = (float)((a + b) * sc);
-      r.scale = (float)sc;
+      r.scale = (float)scale;
       static_cast<B1Bin*>(out)[u] = r;
     } else {
       B0Bin r{};

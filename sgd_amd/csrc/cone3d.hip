@@ -354,8 +354,7 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(Cone3DArgs a, int p0, int
         else if (q0 <= q1) load_col(m + 1, q0, ynext);
         float w;
         if (M == CONE_B1) {
-          w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.nk2, rec.h4),
-                        s_b1(tb, rec.apb, rec.b, rec.a, rec.nk2, rec.h4), ta, tb) * rec.scale;
+          w = b1_mean(rec, ta, tb) * rec.scale;
         } else {
           const int ix = rows ? n : m, iy = rows ? m : n;
           w = sf_cell_weight(vg, g, u, ix, iy);
@@ -436,8 +435,7 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
     if (M == CONE_B1) {
       const B1Bin rec = bins[u];
       const float ta = rec.ka * na, tb = rec.kb * nb;
-      w = mean_over(s_b1(ta, rec.apb, rec.b, rec.a, rec.nk2, rec.h4),
-                    s_b1(tb, rec.apb, rec.b, rec.a, rec.nk2, rec.h4), ta, tb) * rec.scale;
+      w = b1_mean(rec, ta, tb) * rec.scale;
     } else {
       w = sf_cell_weight(vg, g, u, ix, iy);
     }

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void matgen_kernel(Proj2DArgs g, unsigned int*
   const double cx = eax + 0.5 * vg.pux, cy = eay + 0.5 * vg.puy, cl = sqrt(cx * cx + cy * cy);
   const float nxa = (float)(fabs(cy) / cl), nya = (float)(fabs(cx) / cl);
   // (ta, tb below are in the basis's scaled offset: compare with dx, dy scaled alike)
-  const float fdx = (float)g.dx * T::scale(rec), fdy = (float)g.dy * T::scale(rec);
+  const float fdx = (float)g.dx * T::tscale(rec), fdy = (float)g.dy * T::tscale(rec);
   // pixel set: corner rule with half-extent h + 1 (the footprint's support
   // is the projection of the basis at the pixel's depth, within one pixel of
   // the perspective corner range); walk columns ix, rows iy inside

@@ -19,13 +19,12 @@ template <int BASIS> struct Traits;
 
 template <> struct Traits<BASIS_B1> {
   using Bin = B1Bin;
-  // (S(tb') - S(ta')) / (tb' - ta'); times scale(r) it is the footprint.
-  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) {
-    float sa = s_b1(ta, r.apb, r.b, r.a, r.nk2, r.h4);
-    float sb = s_b1(tb, r.apb, r.b, r.a, r.nk2, r.h4);
-    return mean_over(sa, sb, ta, tb);
-  }
+  // times scale(r) it is the footprint
+  __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) { return b1_mean(r, ta, tb); }
   __host__ __device__ static __forceinline__ float scale(const Bin& r) { return r.scale; }
+  // s of the scaled offset t' = s t: scale = s c' and h = 1 / (2 c') (the
+  // tri_a * box record: scale = s, h = 1/2)
+  __host__ __device__ static __forceinline__ float tscale(const Bin& r) { return 2.0f * r.h * r.scale; }
 };
 
 template <> struct Traits<BASIS_B0> {
@@ -36,6 +35,7 @@ template <> struct Traits<BASIS_B0> {
     return mean_over(sa, sb, ta, tb);
   }
   __host__ __device__ static __forceinline__ float scale(const Bin&) { return 1.0f; }
+  __host__ __device__ static __forceinline__ float tscale(const Bin&) { return 1.0f; }
 };
 
 // ---------------------------------------------------------------------------
