@@ -9,7 +9,9 @@ phantom; A^T is applied to its Poisson-noisy (1e5 photons) log sinogram.
 Prints one JSON line (rank 0).  `roofline` prices the dominant kernel with the
 view-visit byte model 4 np (nx ny + nu) bytes per launch (every view reads or
 updates every pixel once and its detector row once), timed per launch with
-HIP events on the launch stream.  `cpu_baseline` times the CPU restatement of
+HIP events on the launch stream; `valu_roofline` prices the same call by its
+VALU instructions against the FP32 issue peak (the resource that binds these
+kernels).  `cpu_baseline` times the CPU restatement of
 the reference formulas (FP32 footprint, FP64 sums, OpenMP) on a bounded
 sample of views and scales to pairs/s.
 """
@@ -40,6 +42,14 @@ HBM_PEAK_GBS = 8000.0
 #   A^T: bwd_pix 73487 x 2 + 32768, sum_parts 16395 x 2 + 1024 KiB
 # (profiles/b1_fanbeam_512x720x768.md).
 TRAFFIC_BYTES_B1 = {"A": 170884 * 1024.0, "A^T": 213556 * 1024.0}
+# VALU wave-instructions per B1 launch at this workload (the projector kernel;
+# the partial-sum kernel adds 4e5 / 2e5), rocprofv3 --pmc SQ_INSTS_VALU
+# (profiles/b1_fanbeam_512x720x768.md).  Peak issue rate of non-packed FP32
+# VALU instructions: 256 CUs x 4 SIMDs x 2.4 GHz / 2 cycles per wave64
+# instruction = 1.2288e12 / s, the FP32 vector peak of 157.3 TFLOP/s divided by
+# the 128 FLOP of one wave64 FMA (packed FP32 is not used).
+VALU_INSTS_B1 = {"A": 7.9743e8, "A^T": 8.1650e8}
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 2
 METRIC = "A+Aᵀ pairs/s at 512²/720v (%HBM peak)"
 
 
@@ -181,6 +191,15 @@ def main():
             "kernel": dom,
             "bytes_per_launch": bytes_per_launch,
         },
+        "valu_roofline": ({
+            "bound": "valu",
+            "achieved": VALU_INSTS_B1[dom] / (dom_ms * 1e-3),
+            "peak": VALU_PEAK_PER_S,
+            "unit": "VALU wave-instructions/s",
+            "frac": VALU_INSTS_B1[dom] / (dom_ms * 1e-3) / VALU_PEAK_PER_S,
+            "insts_per_launch": VALU_INSTS_B1[dom],
+            "kernel": dom,
+        } if args.basis == "B1" else None),
         "kernel_ms": {"A": fwd_ms, "A^T": bwd_ms},
         "footprint_evals_per_A": evals,
     }

@@ -82,11 +82,30 @@ int hl_backward_projection(double *b, double *x, sgdct_projector *h);
  * b1_forward_projection: the reference's matrix kernel averages over
  * [y, y + vblur] (A_B1.cu:402-411), which returns bin u-1's value in row u;
  * its projector and its quadrature generator python/matrix_gen/
- * A_B1_ref_project.py agree with the rows returned here. */
+ * A_B1_ref_project.py agree with the rows returned here.  Addition: with
+ * spmat == NULL and buffer_size == 0 (this and the exporters below) the call
+ * only counts, and returns the number of triplets the matrix needs. */
 int pixel_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
                             double lso, double *spmat, int buffer_size);
 int bilinear_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
                                double lso, double *spmat, int buffer_size);
+
+/* The hexagonal exporters python/matrix_gen/A_H0_project.py:17-19 and
+ * A_Hl_project.py:17-19 bind (h0_matrix_generation; hex_matrix_generation for
+ * Hl).  The reference's own versions are commented out (A_H0.cu:510-553,
+ * A_Hl.cu:521-565; their kernels A_H0.cu:360, A_Hl.cu:370-451 are never
+ * launched).  nl rows x nc columns of lattice constant dt, np views over
+ * [0, 2 pi (np-1)/np]; COO triplets (p*nu + u, il*nc + ic, conv) in the order
+ * view, bin, il, ic, kept where conv > 0, conv being the footprint of
+ * h0_/hl_forward_projection without the dx dy factor (H0 with the 1/3 and
+ * vertex-bound fixes above).  Returns the triplet count or a negative code
+ * (also when more than buffer_size triplets are needed). */
+int h0_matrix_generation(int np, int nu, int nl, int nc, double du, double dt, double lsd, double lso,
+                         double *spmat, int buffer_size);
+int hl_matrix_generation(int np, int nu, int nl, int nc, double du, double dt, double lsd, double lso,
+                         double *spmat, int buffer_size);
+int hex_matrix_generation(int np, int nu, int nl, int nc, double du, double dt, double lsd, double lso,
+                          double *spmat, int buffer_size);
 
 /* layers/A_B1_3D.hh:24-26 / A_B1_3D.cu:701 (b1_3d_init), :709 (forward),
  * :744 (backward).  Cone beam, in-plane B1 footprint times an axial

@@ -114,6 +114,10 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 5 + [_dp, ctypes.c_int]
         f.restype = ctypes.c_int
+    for name in ("h0_matrix_generation", "hl_matrix_generation", "hex_matrix_generation"):
+        f = getattr(L, name)
+        f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 4 + [_dp, ctypes.c_int]
+        f.restype = ctypes.c_int
     L.sgdct_batch_create.argtypes = [ctypes.POINTER(_h), ctypes.c_int]
     L.sgdct_batch_create.restype = _h
     L.sgdct_batch_destroy.argtypes = [_h]

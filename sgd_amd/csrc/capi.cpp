@@ -49,6 +49,8 @@ hipError_t launch_cone_backward_chunk(int model, const Cone3DArgs& a, int p0, in
 constexpr int kMaxNz3D = 512, kMaxNv3D = 512;
 hipError_t launch_matgen(int basis, const Proj2DArgs& g, bool write, unsigned int* counts, const long long* offsets,
                          double* out, hipStream_t s);
+hipError_t launch_hex_matgen(int basis, const Proj2DArgs& g, double dt, bool write, unsigned int* counts,
+                             const long long* offsets, double* out, hipStream_t s);
 hipError_t launch_batch_forward(const BatchPlan& P, int blocks, const float* img, float* sino, hipStream_t s);
 hipError_t nn_conv3x3(const void* in, const void* w, const float* bias, void* out, int B, int H, int W, int cin,
                       int cout, int relu, int out_f32, hipStream_t s);
@@ -795,22 +797,27 @@ int batch_run(sgdct_batch* b, int dir, const float* in, float* out, hipStream_t 
 // order.  Returns the triplet count or a negative code; a buffer smaller than
 // the count is an error (the reference asserts, A_B1.cu:578-579).
 int matrix_generation(int basis, int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
-                      double lso, double* spmat, int buffer_size) {
-  if (!spmat || buffer_size < 0) return fail(SGDCT_EINVAL, "null or negative matrix buffer");
+                      double lso, double* spmat, int buffer_size, double dt = 0.0) {
+  const bool query = !spmat && buffer_size == 0;  // count only
+  if ((!spmat && !query) || buffer_size < 0) return fail(SGDCT_EINVAL, "null or negative matrix buffer");
   if (np <= 0) return fail(SGDCT_EINVAL, "np must be positive");
   std::vector<double> angles(np);
   const double end = M_PI * 2 * (np - 1) / np;
   const double dangle = np > 1 ? end / (np - 1) : 0.0;
   for (int p = 0; p < np; ++p) angles[p] = p * dangle;
-  sgdct_projector* h = create(basis, nx, ny, np, nu, dx, dy, du, lsd, lso, angles.data(), nullptr);
+  sgdct_projector* h = create(basis, nx, ny, np, nu, dx, dy, du, lsd, lso, angles.data(), nullptr, dt);
   if (!h) return SGDCT_EINVAL;
+  auto launch = [&](bool write, unsigned int* counts, const long long* off, double* out) {
+    return is_hex(basis) ? launch_hex_matgen(basis, h->args, dt, write, counts, off, out, nullptr)
+                         : launch_matgen(basis, h->args, write, counts, off, out, nullptr);
+  };
   const size_t nb = (size_t)np * nu;
   unsigned int* d_counts = nullptr;
   long long* d_off = nullptr;
   double* d_out = nullptr;
   int rc = SGDCT_OK;
   hipError_t e = hipMalloc(&d_counts, nb * sizeof(unsigned int));
-  if (e == hipSuccess) e = launch_matgen(basis, h->args, false, d_counts, nullptr, nullptr, nullptr);
+  if (e == hipSuccess) e = launch(false, d_counts, nullptr, nullptr);
   std::vector<unsigned int> counts(nb);
   if (e == hipSuccess) e = hipMemcpy(counts.data(), d_counts, nb * sizeof(unsigned int), hipMemcpyDeviceToHost);
   long long total = 0;
@@ -820,14 +827,18 @@ int matrix_generation(int basis, int np, int nu, int nx, int ny, double dx, doub
       off[i] = total;
       total += counts[i];
     }
-    if (total > (long long)buffer_size) {
+    if (total > (long long)INT_MAX) {
+      rc = fail(SGDCT_EUNSUP, "matrix has " + std::to_string(total) + " triplets, more than an int counts");
+    } else if (query) {
+      rc = SGDCT_OK;
+    } else if (total > (long long)buffer_size) {
       rc = fail(SGDCT_EINVAL, "matrix buffer holds " + std::to_string(buffer_size) + " triplets, " +
                                   std::to_string(total) + " needed");
     } else if (total > 0) {
       e = hipMalloc(&d_off, nb * sizeof(long long));
       if (e == hipSuccess) e = hipMemcpy(d_off, off.data(), nb * sizeof(long long), hipMemcpyHostToDevice);
       if (e == hipSuccess) e = hipMalloc(&d_out, (size_t)total * 3 * sizeof(double));
-      if (e == hipSuccess) e = launch_matgen(basis, h->args, true, d_counts, d_off, d_out, nullptr);
+      if (e == hipSuccess) e = launch(true, d_counts, d_off, d_out);
       if (e == hipSuccess) e = hipMemcpy(spmat, d_out, (size_t)total * 3 * sizeof(double), hipMemcpyDeviceToHost);
     }
   }
@@ -907,6 +918,29 @@ int pixel_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy
 int bilinear_matrix_generation(int np, int nu, int nx, int ny, double dx, double dy, double du, double lsd,
                                double lso, double* spmat, int buffer_size) {
   return matrix_generation(BASIS_B1, np, nu, nx, ny, dx, dy, du, lsd, lso, spmat, buffer_size);
+}
+
+// python/matrix_gen/A_H0_project.py and A_Hl_project.py bind these (the
+// reference's own exporters, A_H0.cu:510-553 and A_Hl.cu:521-565, are
+// commented out): nl rows x nc columns of lattice constant dt, np views over
+// [0, 2 pi (np-1)/np], COO triplets (p*nu + u, il*nc + ic, conv).
+int h0_matrix_generation(int np, int nu, int nl, int nc, double du, double dt, double lsd, double lso,
+                         double* spmat, int buffer_size) {
+  if (!(dt > 0)) return fail(SGDCT_EINVAL, "dt must be positive");
+  return matrix_generation(BASIS_H0, np, nu, nc, nl, dt * std::sqrt(3.0), 1.5 * dt, du, lsd, lso, spmat,
+                           buffer_size, dt);
+}
+
+int hl_matrix_generation(int np, int nu, int nl, int nc, double du, double dt, double lsd, double lso,
+                         double* spmat, int buffer_size) {
+  if (!(dt > 0)) return fail(SGDCT_EINVAL, "dt must be positive");
+  return matrix_generation(BASIS_HL, np, nu, nc, nl, dt * std::sqrt(3.0), 0.5 * dt, du, lsd, lso, spmat,
+                           buffer_size, dt);
+}
+
+int hex_matrix_generation(int np, int nu, int nl, int nc, double du, double dt, double lsd, double lso,
+                          double* spmat, int buffer_size) {
+  return hl_matrix_generation(np, nu, nl, nc, du, dt, lsd, lso, spmat, buffer_size);
 }
 
 // layers/A_B1_3D.hh:24-26 / A_B1_3D.cu:701-770.

@@ -463,6 +463,89 @@ hipError_t launch_hex_backward(int basis, const Proj2DArgs& g, double dt, const 
   return hipGetLastError();
 }
 
+// Explicit system matrix (h0_matrix_generation / hl_matrix_generation; the
+// reference's hl_matrix_generate, layers/A_Hl.cu:370-451, and its H0 twin,
+// A_H0.cu:360, are never called): one lane per (view, bin) walks every
+// lattice row the bin's strip can reach, points in ascending column order,
+// with the same pixel-set tests and footprint as hex_fwd_kernel, and keeps
+// the points whose footprint is > 0 as (p nu + u, il nc + ic, conv) --
+// without the dx dy factor, as the reference's generators.  A count pass,
+// host offsets, then a write pass that never writes past its count.
+template <int HB, bool WRITE>
+__global__ __launch_bounds__(256) void hex_matgen_kernel(HexArgs ha, unsigned int* __restrict__ counts,
+                                                         const long long* __restrict__ offsets,
+                                                         double* __restrict__ out) {
+  using T = HexTraits<HB>;
+  const Proj2DArgs& g = ha.g;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int v = wave / g.nchunk;
+  if (v >= g.np) return;
+  const int u = (wave - v * g.nchunk) * 64 + lane;
+  if (u >= g.nu) return;
+  const ViewGeo vg = g.views[v];
+  const typename T::Bin rec = reinterpret_cast<const typename T::Bin*>(g.bins)[(size_t)v * g.nu + u];
+  const double eax = vg.e0x + u * vg.pux, eay = vg.e0y + u * vg.puy;
+  const double ebx = eax + vg.pux, eby = eay + vg.puy;
+  const double sa = -eay * g.dx, ma = eax * g.dy, ca = eax * vg.w0y - eay * vg.w0x;
+  const double sb = -eby * g.dx, mb = ebx * g.dy, cb = ebx * vg.w0y - eby * vg.w0x;
+  const double lax = (u == 0) ? eax - vg.pux : eax, lay = (u == 0) ? eay - vg.puy : eay;
+  const double lbx = (u == g.nu - 1) ? ebx + vg.pux : ebx, lby = (u == g.nu - 1) ? eby + vg.puy : eby;
+  const double sq = -lay * g.dx, mq = lax * g.dy, cq = lax * vg.w0y - lay * vg.w0x;
+  const double sr = -lby * g.dx, mr = lbx * g.dy, cr = lbx * vg.w0y - lby * vg.w0x;
+  const double rq = hex_reach(-lay, lax, ha.dt), rr = hex_reach(-lby, lbx, ha.dt);
+  const double ka = rec.ka, kb = rec.kb;
+  const double isq = sq != 0.0 ? 1.0 / sq : 0.0, isr = sr != 0.0 ? 1.0 / sr : 0.0;
+  const size_t bin = (size_t)v * g.nu + u;
+  const long long pos = WRITE ? offsets[bin] : 0;
+  const unsigned int cap = WRITE ? counts[bin] : 0u;
+  unsigned int cnt = 0;
+  for (int il = 0; il < g.ny; ++il) {
+    const double sh = (il & 1) ? 0.5 : 0.0;
+    const double r1 = -rq - (mq * il + cq) - sq * sh;
+    const double r2 = rr - (mr * il + cr) - sr * sh;
+    double lo = 0.0, hi = (double)(g.nx - 1 - (il & 1));
+    if (sq > 0.0) lo = fmax(lo, ceil(r1 * isq));
+    else if (sq < 0.0) hi = fmin(hi, floor(r1 * isq));
+    else if (r1 > 0.0) continue;
+    if (sr > 0.0) hi = fmin(hi, ceil(r2 * isr) - 1.0);
+    else if (sr < 0.0) lo = fmax(lo, floor(r2 * isr) + 1.0);
+    else if (r2 <= 0.0) continue;
+    for (int ic = (int)lo; ic <= (int)hi; ++ic) {
+      const double xc = ic + sh;
+      const float ta = (float)(ka * (sa * xc + ma * il + ca));
+      const float tb = (float)(kb * (sb * xc + mb * il + cb));
+      const float w = T::conv(rec, ta, tb);
+      if (!(w > 0.0f)) continue;
+      if (WRITE) {
+        if (cnt >= cap) return;
+        double* o = out + 3 * (pos + cnt);
+        o[0] = (double)bin;
+        o[1] = (double)il * g.nx + ic;
+        o[2] = (double)w;
+      }
+      ++cnt;
+    }
+  }
+  if (!WRITE) counts[bin] = cnt;
+}
+
+hipError_t launch_hex_matgen(int basis, const Proj2DArgs& g, double dt, bool write, unsigned int* counts,
+                             const long long* offsets, double* out, hipStream_t s) {
+  HexArgs ha{g, dt};
+  dim3 grid((g.np * g.nchunk + 3) / 4), block(256);
+  if (basis == BASIS_H0) {
+    if (write) hex_matgen_kernel<BASIS_H0, true><<<grid, block, 0, s>>>(ha, counts, offsets, out);
+    else hex_matgen_kernel<BASIS_H0, false><<<grid, block, 0, s>>>(ha, counts, offsets, out);
+  } else if (basis == BASIS_HL) {
+    if (write) hex_matgen_kernel<BASIS_HL, true><<<grid, block, 0, s>>>(ha, counts, offsets, out);
+    else hex_matgen_kernel<BASIS_HL, false><<<grid, block, 0, s>>>(ha, counts, offsets, out);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_hex_count(const Proj2DArgs& g, double dt, unsigned long long* out, hipStream_t s) {
   HexArgs ha{g, dt};
   hex_count_kernel<<<dim3((g.nx + 15) / 16, (g.ny + 15) / 16), dim3(256), 0, s>>>(ha, out);
