@@ -284,3 +284,44 @@ def sirt_batch(P, b, niter, weights=None, x0=None, lam=1.0):
     for _ in range(niter):
         sirt_step(P, w, x, b, lam, positivity=True)
     return x
+
+
+class CglsResult:
+    def __init__(self, x, residuals, seconds):
+        self.x, self.residuals, self.seconds = x, residuals, seconds
+
+
+def cgls(P, b, niter, x0=None, callback=None):
+    """CGLS (Hestenes-Stiefel conjugate gradients on A^T A x = A^T b, in the
+    least-squares form of Paige & Saunders): the Krylov iterates that LSQR --
+    what pylops' regularized_inversion runs for the reference's notebook
+    reconstructions (python/main copy 2.ipynb cells 14, 27-28) -- produces in
+    exact arithmetic.  Vectors FP32 on the device, inner products in FP64.
+    callback(k, x) after iteration k; returns the iterate and the relative
+    residuals ||b - A x_k|| / ||b||, k = 0..niter."""
+    t0 = time.perf_counter()
+    b = b.contiguous().float()
+    x = torch.zeros(P.img_shape, device=b.device) if x0 is None else x0.clone().contiguous().float()
+    r = b - P.forward(x) if x0 is not None else b.clone()
+    s = P.backward(r)
+    p = s.clone()
+    gamma = float(torch.dot(s.double().ravel(), s.double().ravel()))
+    bn = float(torch.linalg.vector_norm(b.double()))
+    hist = [float(torch.linalg.vector_norm(r.double())) / bn]
+    for k in range(1, niter + 1):
+        q = P.forward(p)
+        qq = float(torch.dot(q.double().ravel(), q.double().ravel()))
+        if qq == 0.0 or gamma == 0.0:
+            break
+        alpha = gamma / qq
+        x.add_(p, alpha=alpha)
+        r.sub_(q, alpha=alpha)
+        s = P.backward(r)
+        gamma_new = float(torch.dot(s.double().ravel(), s.double().ravel()))
+        p.mul_(gamma_new / gamma).add_(s)
+        gamma = gamma_new
+        hist.append(float(torch.linalg.vector_norm(r.double())) / bn)
+        if callback is not None:
+            callback(k, x)
+    torch.cuda.synchronize()
+    return CglsResult(x, hist, time.perf_counter() - t0)
