@@ -189,7 +189,7 @@ int sgdct_project_dev_noalloc(sgdct_projector *h, int direction, const float *in
  * in a fixed order (results are deterministic, independent of the batch size,
  * and agree across splits to FP64 rounding).  0 = automatic: chosen at
  * creation from one image's launch size (a launch of few waves splits, up to
- * 16 ways; hexagonal A stays unsplit); 1..16 = fixed.  The partial-sum workspace (2 images per launch,
+ * 24 ways; hexagonal A stays unsplit); 1..32 = fixed.  The partial-sum workspace (2 images per launch,
  * or up to 16 while it stays within 64 MiB; larger batches take several
  * launches) is allocated at creation and by this call, never inside a
  * projection.  Returns SGDCT_OK, SGDCT_EINVAL or SGDCT_ENOMEM. */

@@ -28,16 +28,20 @@
 
 namespace sgdct {
 
-// grid (waves / 4, batch, nseg).  With nseg > 1 every (segment, image)
-// writes FP64 partial sums to part[(seg * batch + image) * np * nu + ...],
-// which sum_parts_kernel adds in segment order.
+// grid (waves, batch, nseg), one wave per workgroup: waves of a view differ
+// in length, and single-wave workgroups let the dispatcher refill each wave
+// slot as soon as it frees (four-wave workgroups waited for their slowest
+// wave: B1 A 1.058 -> 0.959 ms at 512 x 512, 720 views, 768 bins).  With
+// nseg > 1 every (segment, image) writes FP64 partial sums to
+// part[(seg * batch + image) * np * nu + ...], which sum_parts_kernel adds in
+// segment order.
 template <int BASIS>
-__global__ __launch_bounds__(256) void fwd_bin_kernel(Proj2DArgs g, const float* __restrict__ img,
-                                                      float* __restrict__ sino, double* __restrict__ part) {
+__global__ __launch_bounds__(64) void fwd_bin_kernel(Proj2DArgs g, const float* __restrict__ img,
+                                                     float* __restrict__ sino, double* __restrict__ part) {
   const int b = blockIdx.y, seg = blockIdx.z, nseg = gridDim.z;
   const size_t nsino = (size_t)g.np * g.nu;
-  fwd_bin_wave<BASIS>(g, img + (size_t)b * g.nx * g.ny, sino + b * nsino, blockIdx.x * 4 + (threadIdx.x >> 6),
-                      seg, nseg, nseg > 1 ? part + ((size_t)seg * gridDim.y + b) * nsino : nullptr);
+  fwd_bin_wave<BASIS>(g, img + (size_t)b * g.nx * g.ny, sino + b * nsino, blockIdx.x, seg, nseg,
+                      nseg > 1 ? part + ((size_t)seg * gridDim.y + b) * nsino : nullptr);
 }
 
 // out[i] = sum_k part[k n + i], k ascending (deterministic).
@@ -127,7 +131,7 @@ hipError_t launch_forward(int basis, const Proj2DArgs& g, const float* img, floa
                           double* part, hipStream_t s) {
   const int waves = g.np * g.nchunk;
   if (nseg < 1 || (nseg > 1 && !part)) return hipErrorInvalidValue;
-  dim3 grid((waves + 3) / 4, batch, nseg), block(256);
+  dim3 grid(waves, batch, nseg), block(64);
   if (basis == BASIS_B1) fwd_bin_kernel<BASIS_B1><<<grid, block, 0, s>>>(g, img, sino, part);
   else if (basis == BASIS_B0) fwd_bin_kernel<BASIS_B0><<<grid, block, 0, s>>>(g, img, sino, part);
   else return hipErrorInvalidValue;

@@ -2,13 +2,14 @@
 720 views, 768 bins (0.8 mm), lso 500 mm, lsd 1000 mm -- exactly what
 bench.py times (sgd_amd.workloads.fan512_inputs: the seed-0 phantom and its
 Poisson-noisy log sinogram at 1e5 photons), at the automatic launch split
-(16 line segments for A, 16 view groups for A^T).
+(24 line segments for A, 24 view groups for A^T).
 
   * A(phantom) and A^T(noisy sinogram) against the FP64 oracle over all 720
     views: <= 1e-5 relative L2, and no farther from the FP32 oracle (the
     reference's own arithmetic) than that is from the FP64 one;
   * adjoint with Gaussian x, y at this size: <= 1e-6;
-  * the split 16 / 16 equals the unsplit 1 / 1 to one FP32 rounding;
+  * the split 24 / 24 (the automatic one) equals the unsplit 1 / 1 to one
+    FP32 rounding;
   * the reference-layout FP64 host entry point (b1_forward_projection,
     layers/A_B1.cu:605-678) returns the device path's values and the
     reference's evaluation count.
@@ -84,7 +85,7 @@ def test_fan512_against_oracle(basis, fan512):
 def test_fan512_adjoint_and_split(basis, cuda):
     import torch
     from sgd_amd import workloads as wl
-    print("fan512 %s: adjoint, split 16/16 vs 1/1" % basis, flush=True)
+    print("fan512 %s: adjoint, split 24/24 vs 1/1" % basis, flush=True)
     P = wl.fan512_projector(basis)
     g = torch.Generator(device=cuda).manual_seed(21)
     x = torch.randn(P.img_shape, device=cuda, generator=g)
@@ -93,17 +94,18 @@ def test_fan512_adjoint_and_split(basis, cuda):
     err = float(abs((Ax * y.double()).sum() - (x.double() * Aty).sum()) / (Ax.norm() * y.double().norm()))
     print("  adjoint %.2e" % err, flush=True)
     assert err <= TOL_ADJ
-    P.set_split(16, 16)
-    y16, x16 = P.forward(x), P.backward(y)
+    P.set_split(24, 24)
+    y24, x24 = P.forward(x), P.backward(y)
     P.set_split(1, 1)
     y1, x1 = P.forward(x), P.backward(y)
     torch.cuda.synchronize()
-    print("  split 16/16 vs 1/1: A %s, A^T %s" % ("bitwise" if torch.equal(y16, y1) else "rounding",
-                                                 "bitwise" if torch.equal(x16, x1) else "rounding"))
-    assert torch.allclose(y16, y1, rtol=2.4e-7, atol=1e-30)
-    assert torch.allclose(x16, x1, rtol=2.4e-7, atol=1e-30)
+    print("  split 24/24 vs 1/1: A %s, A^T %s" % ("bitwise" if torch.equal(y24, y1) else "rounding",
+                                                 "bitwise" if torch.equal(x24, x1) else "rounding"))
+    assert torch.allclose(y24, y1, rtol=2.4e-7, atol=1e-30)
+    assert torch.allclose(x24, x1, rtol=2.4e-7, atol=1e-30)
+    # the automatic split at this size is 24 segments / 24 view groups
     P.set_split(0, 0)
-    assert torch.equal(P.forward(x), y16) and torch.equal(P.backward(y), x16)
+    assert torch.equal(P.forward(x), y24) and torch.equal(P.backward(y), x24)
 
 
 def test_fan512_host_abi(fan512):

@@ -142,7 +142,7 @@ def test_autograd_hexagonal_batch(basis):
 
 @pytest.mark.parametrize("basis", ["H0", "Hl"])
 def test_hex_split_agrees(basis):
-    """A with 1..16 row segments and A^T with 1..16 view groups (FP64 partial
+    """A with 1..32 row segments and A^T with 1..32 view groups (FP64 partial
     sums added in a fixed order) give the unsplit results to one FP32
     rounding, each split is deterministic and a batch gets each image's own
     result."""
@@ -153,7 +153,7 @@ def test_hex_split_agrees(basis):
     y = torch.rand((3,) + tuple(P.sino_shape), device="cuda", generator=g)
     P.set_split(1, 1)
     y1, x1 = P.forward(x), P.backward(y)
-    for s in (2, 3, 7, 16, 0):
+    for s in (2, 3, 7, 16, 32, 0):
         P.set_split(s, s)
         ys, xs = P.forward(x), P.backward(y)
         torch.cuda.synchronize()

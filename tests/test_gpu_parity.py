@@ -136,7 +136,7 @@ def test_fan256_full_size(basis, cuda):
 
 @pytest.mark.parametrize("basis", ["B0", "B1", "SF"])
 def test_split_launches_agree(basis, cuda):
-    """A with 1..16 line segments and A^T with 1..16 view groups (FP64
+    """A with 1..32 line segments and A^T with 1..32 view groups (FP64
     partial sums added in a fixed order) give the unsplit result to one FP32
     rounding, and each split is deterministic."""
     import torch
@@ -146,7 +146,7 @@ def test_split_launches_agree(basis, cuda):
     y = torch.rand(P.sino_shape, device="cuda", generator=g)
     P.set_split(1, 1)
     y1, x1 = P.forward(x), P.backward(y)
-    for fs, bs in ((2, 3), (7, 5), (16, 16), (0, 0)):
+    for fs, bs in ((2, 3), (7, 5), (16, 16), (24, 32), (0, 0)):
         P.set_split(fs, bs)
         ys, xs = P.forward(x), P.backward(y)
         torch.cuda.synchronize()

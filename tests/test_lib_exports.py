@@ -33,11 +33,11 @@ def test_binding_sets_signatures():
 
 def test_split_config_rejects_bad_arguments_without_a_device():
     """Argument checks run before any device work: a null handle or a split
-    outside 0..16 is SGDCT_EINVAL with a message."""
+    outside 0..32 is SGDCT_EINVAL with a message."""
     L = _lib.lib()
     assert L.sgdct_split_config(None, 0, 0) == -1
     assert "split" in _lib.last_error()
-    assert L.sgdct_split_config(None, 17, 0) == -1
+    assert L.sgdct_split_config(None, 33, 0) == -1
 
 
 def test_tv_descent_rejects_bad_arguments_without_a_device():
