@@ -38,17 +38,18 @@ HBM_PEAK_GBS = 8000.0
 # (the projector kernel and the partial-sum kernel), from rocprofv3 --pmc
 # FETCH_SIZE then WRITE_SIZE (KiB per dispatch; FETCH_SIZE scaled by 2.00 and
 # WRITE_SIZE by 1.00, the factors a 256 MiB device copy gives, --copy-calib):
-#   A:   fwd_bin 15231 x 2 + 69120, sum_parts 34571 x 2 + 2160 KiB
-#   A^T: bwd_pix 73487 x 2 + 32768, sum_parts 16395 x 2 + 1024 KiB
-# (profiles/b1_fanbeam_512x720x768.md).
-TRAFFIC_BYTES_B1 = {"A": 170884 * 1024.0, "A^T": 213556 * 1024.0}
+#   A:   fwd_bin 16744 x 2 + 103680, sum_parts 51851 x 2 + 2160 KiB
+#   A^T: bwd_pix 73486 x 2 + 49152, sum_parts 24587 x 2 + 1024 KiB
+# (24 line segments / 24 view groups, each writing FP64 partial sums;
+# profiles/b1_fanbeam_512x720x768.md).
+TRAFFIC_BYTES_B1 = {"A": 243030 * 1024.0, "A^T": 246322 * 1024.0}
 # VALU wave-instructions per B1 launch at this workload (the projector kernel;
-# the partial-sum kernel adds 4e5 / 2e5), rocprofv3 --pmc SQ_INSTS_VALU
+# the partial-sum kernel adds 5e5 / 3e5), rocprofv3 --pmc SQ_INSTS_VALU
 # (profiles/b1_fanbeam_512x720x768.md).  Peak issue rate of non-packed FP32
 # VALU instructions: 256 CUs x 4 SIMDs x 2.4 GHz / 2 cycles per wave64
 # instruction = 1.2288e12 / s, the FP32 vector peak of 157.3 TFLOP/s divided by
 # the 128 FLOP of one wave64 FMA (packed FP32 is not used).
-VALU_INSTS_B1 = {"A": 7.9743e8, "A^T": 8.1650e8}
+VALU_INSTS_B1 = {"A": 7.8705e8, "A^T": 8.1794e8}
 VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 2
 METRIC = "A+Aᵀ pairs/s at 512²/720v (%HBM peak)"
 
