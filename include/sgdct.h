@@ -114,7 +114,8 @@ int hex_matrix_generation(int np, int nu, int nl, int nc, double du, double dt, 
  * evaluation count; a value >= 2^63 is a negative SGDCT_E* code.  The
  * reference's back-projector returns zeros (A_B1_3D.cu:408, 436 multiply by
  * the zeroed output); this one is the transpose of the forward and returns
- * SGDCT_OK or a negative code.  nz <= 512 and nv <= 512. */
+ * SGDCT_OK or a negative code.  Any nz and nv (columns of more than 512
+ * voxels and detectors of more than 512 rows are processed in chunks). */
 sgdct_projector *b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, double dx, double dy,
                             double dz, double du, double dv, double lsd, double lso,
                             const double *angles);

@@ -46,7 +46,6 @@ hipError_t launch_cone_forward_chunk(int model, const Cone3DArgs& a, int p0, int
                                      float* proj, hipStream_t s);
 hipError_t launch_cone_backward_chunk(int model, const Cone3DArgs& a, int p0, int nview, const float* projT,
                                       float* Yb, float* xT, int accumulate, hipStream_t s);
-constexpr int kMaxNz3D = 512, kMaxNv3D = 512;
 hipError_t launch_matgen(int basis, const Proj2DArgs& g, bool write, unsigned int* counts, const long long* offsets,
                          double* out, hipStream_t s);
 hipError_t launch_hex_matgen(int basis, const Proj2DArgs& g, double dt, bool write, unsigned int* counts,
@@ -954,10 +953,6 @@ sgdct_projector* b1_3d_init(int nx, int ny, int nz, int np, int nu, int nv, doub
     fail(SGDCT_EINVAL, "invalid cone-beam geometry arguments");
     return nullptr;
   }
-  if (nz > kMaxNz3D || nv > kMaxNv3D) {
-    fail(SGDCT_EUNSUP, "cone-beam projector supports nz <= 512 and nv <= 512");
-    return nullptr;
-  }
   sgdct_projector* h = create(BASIS_B1, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, nullptr);
   if (!h) return nullptr;
   release_part(h);  // the cone-beam kernels do not split their launches
@@ -983,10 +978,6 @@ sgdct_projector* sf3d_init(int nx, int ny, int nz, int np, int nu, int nv, doubl
                            double dv, double lsd, double lso, const double* angles, int trap_v) {
   if (nz <= 0 || nv <= 0 || !(dz > 0) || !(dv > 0)) {
     fail(SGDCT_EINVAL, "invalid cone-beam geometry arguments");
-    return nullptr;
-  }
-  if (nz > kMaxNz3D || nv > kMaxNv3D) {
-    fail(SGDCT_EUNSUP, "cone-beam projector supports nz <= 512 and nv <= 512");
     return nullptr;
   }
   sgdct_projector* h = create(BASIS_SF, nx, ny, np, nu, dx, dy, du, lsd, lso, angles, nullptr, 0.0, 0.5);

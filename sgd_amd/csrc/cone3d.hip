@@ -37,6 +37,7 @@
 // coordinates, so "bin u" of the in-plane machinery is cell [u - 1/2, u + 1/2].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -46,8 +47,12 @@
 
 namespace sgdct {
 
-constexpr int kMaxNz = 512;        // voxels per column staged in LDS
-constexpr int kMaxNv = 512;        // detector rows of a row profile staged in LDS
+// Columns taller than kMaxNz voxels are processed in chunks of kMaxNz (the
+// axial forward stages a chunk's voxel records in LDS, the axial transpose
+// holds a chunk's voxel sums in registers); detector rows are staged for the
+// axial transpose in windows of kMaxNv.  Neither bounds nz or nv.
+constexpr int kMaxNz = 512;
+constexpr int kMaxNv = 512;
 // The in-plane kernels give each wave RCW chunks of 64 detector rows (a grid
 // dimension runs over the chunk groups): with RCW < nv / 64 more waves are in
 // flight on the row profiles, at the price of recomputing the in-plane weight
@@ -157,9 +162,19 @@ __device__ __forceinline__ bool keep_gamma(float gm) {
   return M == CONE_B1 ? gm > kGammaEps : gm == gm;
 }
 
+// Orders a wave's LDS accesses across lanes: LDS operations of one wave are
+// performed in program order, so a compiler barrier at wavefront scope suffices.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // Y[pl][c][j] = sum_iz xT[c][iz] * cfac * gamma_j.  Block: 4 waves, one
-// column each; grid (ceil(ncol / 4), views in the chunk).
+// column each (each wave uses only its own LDS records); grid
+// (ceil(ncol / 4), views in the chunk).  BIG (nz > kMaxNz): the column runs
+// in chunks of kMaxNz voxels, the rows' sums added chunk by chunk.
 // ---------------------------------------------------------------------------
 // Dynamic LDS: per wave and voxel a record of kAxialFields floats (v0, v1,
 // v2, v3, i01, i23, x * cf).  The odd stride puts the records of 64
@@ -167,32 +182,18 @@ __device__ __forceinline__ bool keep_gamma(float gm) {
 // neighbouring voxels) do not conflict.
 constexpr int kAxialFields = 7;
 
-template <int M>
+template <int M, bool BIG>
 __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, const float* __restrict__ xT,
                                                         float* __restrict__ Y) {
   extern __shared__ float axial_lds[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  float* const R = axial_lds + (size_t)wv * kAxialFields * a.nz;  // this wave's records
+  const int zcap = BIG ? kMaxNz : a.nz;
+  float* const R = axial_lds + (size_t)wv * kAxialFields * zcap;  // this wave's records
   const int ncol = a.g.nx * a.g.ny;
   const int c = blockIdx.x * 4 + wv, pl = blockIdx.y;
   const bool live = c < ncol;
   ColumnAxial ca{};
-  if (live) {
-    ca = column_axial<M>(a, a.g.views[p0 + pl], c % a.g.nx, c / a.g.nx);
-    for (int iz = lane; iz < a.nz; iz += 64) {
-      const AxialTrap t = voxel_trap<M>(a, ca, iz);
-      float* const r = R + kAxialFields * iz;
-      r[0] = t.v0;
-      r[1] = t.v1;
-      r[2] = t.v2;
-      r[3] = t.v3;
-      r[4] = t.i01;
-      r[5] = t.i23;
-      r[6] = xT[(size_t)c * a.nz + iz] * t.cf;
-    }
-  }
-  __syncthreads();
-  if (!live) return;
+  if (live) ca = column_axial<M>(a, a.g.views[p0 + pl], c % a.g.nx, c / a.g.nx);
   float* yrow = Y + ((size_t)pl * ncol + c) * a.nv;
   // v3(iz) = vc + k (zc + 1/2), zc = iz - (nz - 1) / 2, with k = max(ka, kb)
   // above vc and min(ka, kb) below; it grows with iz (ka, kb > 0: voxels in
@@ -200,28 +201,56 @@ __global__ __launch_bounds__(256) void axial_fwd_kernel(Cone3DArgs a, int p0, co
   // line up to rounding, and the staged values settle it exactly
   const float ikhi = 1.0f / fmaxf(ca.ka, ca.kb), iklo = 1.0f / fminf(ca.ka, ca.kb);
   const float zoff = 0.5f * (float)(a.nz - 1) - 0.5f;
-  for (int j = lane; j < a.nv; j += 64) {
-    const float fj = (float)j + row_off<M>();
-    // first voxel whose footprint reaches row j
-    const float est = __builtin_fmaf(fj - a.vc, fj >= a.vc ? ikhi : iklo, zoff);
-    int lo = (int)fminf(fmaxf(ceilf(est), 0.0f), (float)a.nz);
-    while (lo > 0 && R[kAxialFields * (lo - 1) + 3] >= fj) --lo;
-    while (lo < a.nz && R[kAxialFields * lo + 3] < fj) ++lo;
-    float acc = 0.0f;
-    for (int iz = lo; iz < a.nz; ++iz) {
-      const float* const r = R + kAxialFields * iz;
-      if (!(r[0] < fj + 1.0f)) break;
-      AxialTrap t;
-      t.v0 = r[0];
-      t.v1 = r[1];
-      t.v2 = r[2];
-      t.v3 = r[3];
-      t.i01 = r[4];
-      t.i23 = r[5];
-      const float gm = trap_gamma(fj, t);
-      if (keep_gamma<M>(gm)) acc = __builtin_fmaf(r[6], gm, acc);
+  // voxels [z0, z1): stage their records, then add their share to every row
+  auto chunk = [&](int z0, int z1, bool first) {
+    if (live) {
+      for (int iz = z0 + lane; iz < z1; iz += 64) {
+        const AxialTrap t = voxel_trap<M>(a, ca, iz);
+        float* const r = R + kAxialFields * (iz - z0);
+        r[0] = t.v0;
+        r[1] = t.v1;
+        r[2] = t.v2;
+        r[3] = t.v3;
+        r[4] = t.i01;
+        r[5] = t.i23;
+        r[6] = xT[(size_t)c * a.nz + iz] * t.cf;
+      }
     }
-    yrow[j] = acc;
+    __syncthreads();
+    if (!live) return;
+    for (int j = lane; j < a.nv; j += 64) {
+      const float fj = (float)j + row_off<M>();
+      // first voxel of the chunk whose footprint reaches row j
+      const float est = __builtin_fmaf(fj - a.vc, fj >= a.vc ? ikhi : iklo, zoff);
+      int lo = (int)fminf(fmaxf(ceilf(est), (float)z0), (float)z1);
+      while (lo > z0 && R[kAxialFields * (lo - 1 - z0) + 3] >= fj) --lo;
+      while (lo < z1 && R[kAxialFields * (lo - z0) + 3] < fj) ++lo;
+      float acc = 0.0f;
+      for (int iz = lo; iz < z1; ++iz) {
+        const float* const r = R + kAxialFields * (iz - z0);
+        if (!(r[0] < fj + 1.0f)) break;
+        AxialTrap t;
+        t.v0 = r[0];
+        t.v1 = r[1];
+        t.v2 = r[2];
+        t.v3 = r[3];
+        t.i01 = r[4];
+        t.i23 = r[5];
+        const float gm = trap_gamma(fj, t);
+        if (keep_gamma<M>(gm)) acc = __builtin_fmaf(r[6], gm, acc);
+      }
+      if (first) yrow[j] = acc;
+      else yrow[j] += acc;
+    }
+  };
+  if (!BIG) {
+    chunk(0, a.nz, true);
+    return;
+  }
+  // every wave of the block reaches both barriers of every chunk
+  for (int z0 = 0; z0 < a.nz; z0 += kMaxNz) {
+    if (z0 > 0) __syncthreads();  // the previous chunk's reads come first
+    chunk(z0, min(a.nz, z0 + kMaxNz), z0 == 0);
   }
 }
 
@@ -457,17 +486,11 @@ __global__ __launch_bounds__(256) void conv_bwd_kernel(Cone3DArgs a, int p0, con
 // Wave per column, lanes over voxels; grid ceil(ncol / 4).  Each view's row
 // profile Yb[c][*] is copied into the wave's LDS with coalesced loads (the
 // next view's loads are in flight meanwhile); the voxels then gather their
-// rows from LDS.
+// rows from LDS.  BIG (nz > kMaxNz or nv > kMaxNv): the grid's y dimension
+// takes the column's voxels in chunks of kMaxNz, and each view's profile is
+// staged kMaxNv rows at a time.
 // ---------------------------------------------------------------------------
-// Orders a wave's LDS accesses across lanes: LDS operations of one wave are
-// performed in program order, so a compiler barrier at wavefront scope suffices.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <int M>
+template <int M, bool BIG>
 __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, int nview, const float* __restrict__ Yb,
                                                         float* __restrict__ xT, int accumulate) {
   __shared__ float prof_lds[4][kMaxNv];
@@ -478,38 +501,50 @@ __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, in
   float* const yl = prof_lds[wv];
   const int ix = c % a.g.nx, iy = c / a.g.nx;
   constexpr int kSlots = kMaxNz / 64, kRowSlots = kMaxNv / 64;
+  const int z0 = BIG ? blockIdx.y * kMaxNz : 0;
+  const int nwin = BIG ? (a.nv + kMaxNv - 1) / kMaxNv : 1;
   double acc[kSlots];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) acc[s] = 0.0;
   float pre[kRowSlots];
-  auto fetch = [&](int pl) {
-    const float* yrow = Yb + ((size_t)pl * ncol + c) * a.nv;
+  // window q = pl * nwin + w: rows [w kMaxNv, min(nv, (w + 1) kMaxNv)) of view pl
+  auto fetch = [&](int q) {
+    const int pl = BIG ? q / nwin : q, r0 = BIG ? (q - pl * nwin) * kMaxNv : 0;
+    const float* yrow = Yb + ((size_t)pl * ncol + c) * a.nv + r0;
+    const int nr = BIG ? min(kMaxNv, a.nv - r0) : a.nv;
 #pragma unroll
     for (int k = 0; k < kRowSlots; ++k)
-      if (k * 64 < a.nv) pre[k] = (k * 64 + lane < a.nv) ? yrow[k * 64 + lane] : 0.0f;
+      if (k * 64 < nr) pre[k] = (k * 64 + lane < nr) ? yrow[k * 64 + lane] : 0.0f;
   };
   fetch(0);
   const float lastj = (float)(a.nv - 1);
-  for (int pl = 0; pl < nview; ++pl) {
-    wave_lds_sync();  // the previous view's reads come first
+  const int nq = nview * nwin;
+  for (int q = 0; q < nq; ++q) {
+    const int pl = BIG ? q / nwin : q, r0 = BIG ? (q - pl * nwin) * kMaxNv : 0;
+    const int nr = BIG ? min(kMaxNv, a.nv - r0) : a.nv;
+    wave_lds_sync();  // the previous window's reads come first
 #pragma unroll
     for (int k = 0; k < kRowSlots; ++k)
-      if (k * 64 + lane < a.nv) yl[k * 64 + lane] = pre[k];
+      if (k * 64 + lane < nr) yl[k * 64 + lane] = pre[k];
     wave_lds_sync();
-    if (pl + 1 < nview) fetch(pl + 1);
+    if (q + 1 < nq) fetch(q + 1);
     const ColumnAxial ca = column_axial<M>(a, a.g.views[p0 + pl], ix, iy);
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
-      const int iz = s * 64 + lane;
+      const int iz = z0 + s * 64 + lane;
       if (iz >= a.nz) continue;
       const AxialTrap t = voxel_trap<M>(a, ca, iz);
       const float ro = row_off<M>();
-      const int jlo = (int)fminf(fmaxf(floorf(t.v0 - ro), 0.0f), lastj);
-      const int jhi = (int)fminf(fmaxf(floorf(t.v3 - ro), 0.0f), lastj);
+      int jlo = (int)fminf(fmaxf(floorf(t.v0 - ro), 0.0f), lastj);
+      int jhi = (int)fminf(fmaxf(floorf(t.v3 - ro), 0.0f), lastj);
+      if (BIG) {
+        jlo = max(jlo, r0);
+        jhi = min(jhi, r0 + nr - 1);
+      }
       float sum = 0.0f;
       for (int j = jlo; j <= jhi; ++j) {
         const float gm = trap_gamma((float)j + ro, t);
-        if (keep_gamma<M>(gm)) sum = __builtin_fmaf(gm, yl[j], sum);
+        if (keep_gamma<M>(gm)) sum = __builtin_fmaf(gm, yl[j - r0], sum);
       }
       acc[s] += (double)(t.cf * sum);
     }
@@ -517,7 +552,7 @@ __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, in
   float* out = xT + (size_t)c * a.nz;
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) {
-    const int iz = s * 64 + lane;
+    const int iz = z0 + s * 64 + lane;
     if (iz < a.nz) out[iz] = (float)((accumulate ? (double)out[iz] : 0.0) + acc[s]);
   }
 }
@@ -552,8 +587,9 @@ template <int M>
 static void cone_forward_chunk(const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y, float* proj,
                                hipStream_t s) {
   const int ncol = a.g.nx * a.g.ny;
-  const size_t lds = sizeof(float) * 4 * kAxialFields * a.nz;  // 4 waves
-  axial_fwd_kernel<M><<<dim3((ncol + 3) / 4, nview), dim3(256), lds, s>>>(a, p0, xT, Y);
+  const size_t lds = sizeof(float) * 4 * kAxialFields * std::min(a.nz, kMaxNz);  // 4 waves
+  if (a.nz > kMaxNz) axial_fwd_kernel<M, true><<<dim3((ncol + 3) / 4, nview), dim3(256), lds, s>>>(a, p0, xT, Y);
+  else axial_fwd_kernel<M, false><<<dim3((ncol + 3) / 4, nview), dim3(256), lds, s>>>(a, p0, xT, Y);
   // every row in one wave: with the next column's profile loaded during the
   // weight computation, splitting the rows over more waves (each computing
   // the weights again) no longer pays (B1, 16 views of 256^2 x 384: 2 chunks
@@ -567,8 +603,13 @@ template <int M>
 static void cone_backward_chunk(const Cone3DArgs& a, int p0, int nview, const float* projT, float* Yb, float* xT,
                                 int accumulate, hipStream_t s) {
   const int ncol = a.g.nx * a.g.ny;
-  conv_bwd_kernel<M, kAllRows><<<dim3((ncol + 3) / 4, nview, 1), dim3(256), 0, s>>>(a, p0, projT, Yb);
-  axial_bwd_kernel<M><<<dim3((ncol + 3) / 4), dim3(256), 0, s>>>(a, p0, nview, Yb, xT, accumulate);
+  const int groups = (a.nrc + kAllRows - 1) / kAllRows;  // 512 detector rows per wave
+  conv_bwd_kernel<M, kAllRows><<<dim3((ncol + 3) / 4, nview, groups), dim3(256), 0, s>>>(a, p0, projT, Yb);
+  const dim3 grid((ncol + 3) / 4, (a.nz + kMaxNz - 1) / kMaxNz);
+  if (a.nv > kMaxNv || a.nz > kMaxNz)
+    axial_bwd_kernel<M, true><<<grid, dim3(256), 0, s>>>(a, p0, nview, Yb, xT, accumulate);
+  else
+    axial_bwd_kernel<M, false><<<grid, dim3(256), 0, s>>>(a, p0, nview, Yb, xT, accumulate);
 }
 
 hipError_t launch_cone_forward_chunk(int model, const Cone3DArgs& a, int p0, int nview, const float* xT, float* Y,

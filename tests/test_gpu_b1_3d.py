@@ -20,9 +20,12 @@ LSD, LSO = 200.0, 100.0
 CASES = {
     "small": dict(nx=24, ny=20, nz=18, np_=12, nu=48, nv=40, d=1.0, du=1.2, dv=1.1),
     "mid": dict(nx=64, ny=60, nz=48, np_=16, nu=128, nv=96, d=0.8, du=1.0, dv=1.0),
-    # the largest column and detector height the kernels stage (512 voxels,
-    # 512 rows)
+    # the largest column and detector height the kernels stage at once (512
+    # voxels, 512 rows)
     "tall": dict(nx=8, ny=6, nz=512, np_=4, nu=24, nv=512, d=0.2, du=0.5, dv=0.4),
+    # past them: columns of 600 voxels (two voxel chunks) on 640 detector rows
+    # (two row windows of the axial transpose, two row groups in-plane)
+    "taller": dict(nx=6, ny=5, nz=600, np_=3, nu=16, nv=640, d=0.2, du=0.5, dv=0.4),
     # a detector three times wider than the volume: the strips of the outer
     # bins miss it, so their walks have no line
     "wide": dict(nx=16, ny=14, nz=10, np_=8, nu=96, nv=24, d=1.0, du=1.0, dv=1.0),
