@@ -267,10 +267,23 @@ int sgdct_batch_graph_launch(sgdct_batch *b, int direction, const float *in, flo
  * out_f32 (the input-gradient pass: in = bf16 output gradient, w = flipped). */
 int sgdct_nn_conv3x3(const void *in, const void *w, const float *bias, void *out, int B, int H, int W, int cin,
                      int cout, int relu, int out_f32, void *stream);
+/* sgdct_nn_conv3x3 with two fusions.  in2 != NULL: the input is the decoder's
+ * concatenation [upsample2(in), in2], read in place: in holds c0 channels at
+ * H/2 x W/2 (nearest-neighbour upsampled), in2 the other cin - c0 channels at
+ * H x W (c0 a multiple of 8, H and W even).  mask != NULL ([B][H][W][cout]
+ * bf16): each output value is kept where mask > 0 and zeroed elsewhere (the
+ * ReLU mask of an activation gradient, applied before the bf16 / FP32 store). */
+int sgdct_nn_conv3x3_ex(const void *in, const void *in2, int c0, const void *w, const float *bias, void *out,
+                        const void *mask, int B, int H, int W, int cin, int cout, int relu, int out_f32,
+                        void *stream);
 long long sgdct_nn_wgrad_workspace(int B, int H, int W, int cin, int cout);
 /* dw[co][k] = sum over pixels of dy[m][co] x[m + tap(k)][c(k)] */
 int sgdct_nn_wgrad(const void *x, const void *dy, float *dw, float *workspace, long long workspace_floats, int B,
                    int H, int W, int cin, int cout, void *stream);
+/* sgdct_nn_wgrad of the concatenated input [upsample2(x), x2] (as
+ * sgdct_nn_conv3x3_ex; x2 = NULL is sgdct_nn_wgrad). */
+int sgdct_nn_wgrad_ex(const void *x, const void *x2, int c0, const void *dy, float *dw, float *workspace,
+                      long long workspace_floats, int B, int H, int W, int cin, int cout, void *stream);
 long long sgdct_nn_colsum_workspace(long long M, int C);
 int sgdct_nn_colsum(const void *a, float *out, float *workspace, long long workspace_floats, long long M, int C,
                     void *stream);

@@ -88,6 +88,8 @@ def lib():
         "sgdct_nn_conv3x3": [_v, _v, _v, _v] + [_i] * 7 + [_v],
         "sgdct_nn_wgrad_workspace": [_i] * 5,
         "sgdct_nn_wgrad": [_v, _v, _v, _v, _ll] + [_i] * 5 + [_v],
+        "sgdct_nn_conv3x3_ex": [_v, _v, _i, _v, _v, _v, _v] + [_i] * 7 + [_v],
+        "sgdct_nn_wgrad_ex": [_v, _v, _i, _v, _v, _v, _ll] + [_i] * 5 + [_v],
         "sgdct_nn_colsum_workspace": [_ll, _i],
         "sgdct_nn_colsum": [_v, _v, _v, _ll, _ll, _i, _v],
         "sgdct_nn_pad_input": [_v, _v, _ll, _v],

@@ -51,11 +51,11 @@ hipError_t launch_matgen(int basis, const Proj2DArgs& g, bool write, unsigned in
 hipError_t launch_hex_matgen(int basis, const Proj2DArgs& g, double dt, bool write, unsigned int* counts,
                              const long long* offsets, double* out, hipStream_t s);
 hipError_t launch_batch_forward(const BatchPlan& P, int blocks, const float* img, float* sino, hipStream_t s);
-hipError_t nn_conv3x3(const void* in, const void* w, const float* bias, void* out, int B, int H, int W, int cin,
-                      int cout, int relu, int out_f32, hipStream_t s);
+hipError_t nn_conv3x3(const void* in, const void* in2, int c0, const void* w, const float* bias, void* out,
+                      const void* mask, int B, int H, int W, int cin, int cout, int relu, int out_f32, hipStream_t s);
 long long nn_wgrad_splits(long long M);
-hipError_t nn_wgrad(const void* x, const void* dy, float* dw, float* partial, int B, int H, int W, int cin, int cout,
-                    hipStream_t s);
+hipError_t nn_wgrad(const void* x, const void* x2, int c0, const void* dy, float* dw, float* partial, int B, int H,
+                    int W, int cin, int cout, hipStream_t s);
 hipError_t nn_colsum_bf16(const void* a, float* out, float* partial, long long M, int C, hipStream_t s);
 hipError_t nn_pad_input(const float* x, void* out8, long long n, hipStream_t s);
 hipError_t nn_pool2(const void* src, void* dst, int B, int H, int W, int C, hipStream_t s);
@@ -1217,9 +1217,26 @@ bool dims_ok(int B, int H, int W) { return B > 0 && H > 0 && W > 0; }
 
 int sgdct_nn_conv3x3(const void* in, const void* w, const float* bias, void* out, int B, int H, int W, int cin,
                      int cout, int relu, int out_f32, void* stream) {
+  return sgdct_nn_conv3x3_ex(in, nullptr, 0, w, bias, out, nullptr, B, H, W, cin, cout, relu, out_f32, stream);
+}
+
+namespace {
+// A two-source input (in at half resolution, c0 channels; in2 the skip
+// tensor): c0 a positive multiple of 8 below cin, H and W even.
+bool concat_ok(const void* in2, int c0, int H, int W, int cin) {
+  return !in2 || (c0 > 0 && c0 < cin && c0 % 8 == 0 && H % 2 == 0 && W % 2 == 0);
+}
+}  // namespace
+
+int sgdct_nn_conv3x3_ex(const void* in, const void* in2, int c0, const void* w, const float* bias, void* out,
+                        const void* mask, int B, int H, int W, int cin, int cout, int relu, int out_f32,
+                        void* stream) {
   if (!in || !w || !out || !dims_ok(B, H, W) || cin <= 0 || cin % 8 || cout <= 0)
     return fail(SGDCT_EINVAL, "bad conv3x3 arguments (cin must be a multiple of 8)");
-  return nn_ret(nn_conv3x3(in, w, bias, out, B, H, W, cin, cout, relu, out_f32, (hipStream_t)stream), "conv3x3");
+  if (!concat_ok(in2, c0, H, W, cin))
+    return fail(SGDCT_EINVAL, "bad conv3x3 concatenation (0 < c0 < cin, c0 % 8 == 0, even H and W)");
+  return nn_ret(nn_conv3x3(in, in2, c0, w, bias, out, mask, B, H, W, cin, cout, relu, out_f32, (hipStream_t)stream),
+                "conv3x3");
 }
 
 long long sgdct_nn_wgrad_workspace(int B, int H, int W, int cin, int cout) {
@@ -1229,11 +1246,18 @@ long long sgdct_nn_wgrad_workspace(int B, int H, int W, int cin, int cout) {
 
 int sgdct_nn_wgrad(const void* x, const void* dy, float* dw, float* workspace, long long workspace_floats, int B,
                    int H, int W, int cin, int cout, void* stream) {
+  return sgdct_nn_wgrad_ex(x, nullptr, 0, dy, dw, workspace, workspace_floats, B, H, W, cin, cout, stream);
+}
+
+int sgdct_nn_wgrad_ex(const void* x, const void* x2, int c0, const void* dy, float* dw, float* workspace,
+                      long long workspace_floats, int B, int H, int W, int cin, int cout, void* stream) {
   if (!x || !dy || !dw || !workspace || !dims_ok(B, H, W) || cin <= 0 || cin % 8 || cout <= 0 || cout % 8)
     return fail(SGDCT_EINVAL, "bad wgrad arguments (cin and cout must be multiples of 8)");
+  if (!concat_ok(x2, c0, H, W, cin))
+    return fail(SGDCT_EINVAL, "bad wgrad concatenation (0 < c0 < cin, c0 % 8 == 0, even H and W)");
   if (workspace_floats < sgdct_nn_wgrad_workspace(B, H, W, cin, cout))
     return fail(SGDCT_EINVAL, "wgrad workspace too small");
-  return nn_ret(nn_wgrad(x, dy, dw, workspace, B, H, W, cin, cout, (hipStream_t)stream), "wgrad");
+  return nn_ret(nn_wgrad(x, x2, c0, dy, dw, workspace, B, H, W, cin, cout, (hipStream_t)stream), "wgrad");
 }
 
 long long sgdct_nn_colsum_workspace(long long M, int C) {

@@ -34,13 +34,28 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 struct ConvArgs {
-  const bf16* in;      // [B][H][W][cin]
+  const bf16* in;      // [B][H][W][cin] (with in2: [B][H/2][W/2][c0])
   const bf16* w;       // [cout][9 cin]
   const float* bias;   // [cout] or null
   void* out;           // [B][H][W][cout], bf16 or FP32
   int B, H, W, cin, cout;
   int relu, out_f32;
+  const bf16* mask;    // [B][H][W][cout] or null: out = value * (mask > 0)
+  const bf16* in2;     // [B][H][W][cin - c0] or null (see conv_in)
+  int c0;
 };
+
+// Address of the 8 input channels c .. c + 7 at pixel (b, y, x).  With in2 the
+// input is the decoder's concatenation, read in place: channels [0, c0) are
+// the half-resolution tensor `in` upsampled by nearest neighbour, channels
+// [c0, cin) the full-resolution skip tensor in2 (c0 % 8 == 0, so a group of
+// 8 never straddles the two).
+__device__ __forceinline__ const bf16* conv_in(const bf16* in, const bf16* in2, int c0, int H, int W, int cin, int b,
+                                               int y, int x, int c) {
+  if (!in2) return in + (((size_t)b * H + y) * W + x) * cin + c;
+  if (c < c0) return in + (((size_t)b * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * c0 + c;
+  return in2 + (((size_t)b * H + y) * W + x) * (cin - c0) + (c - c0);
+}
 
 __device__ __forceinline__ bf16x8 zero8() {
   bf16x8 z;
@@ -98,7 +113,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
     for (int i = 0; i < 2; ++i) {
       const int yy = py[i] + dy, xx = px[i] + dx;
       if (kv && pv[i] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-        av[i] = *reinterpret_cast<const bf16x8*>(a.in + (((size_t)pb[i] * a.H + yy) * a.W + xx) * a.cin + c);
+        av[i] = *reinterpret_cast<const bf16x8*>(conv_in(a.in, a.in2, a.c0, a.H, a.W, a.cin, pb[i], yy, xx, c));
       else
         av[i] = zero8();
     }
@@ -128,6 +143,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
         if (m >= M) continue;
         float v = acc[i][j][r] + bias;
         if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.mask && !((float)a.mask[m * a.cout + co[j]] > 0.0f)) v = 0.0f;
         if (a.out_f32) static_cast<float*>(a.out)[m * a.cout + co[j]] = v;
         else static_cast<bf16*>(a.out)[m * a.cout + co[j]] = (bf16)v;
       }
@@ -190,7 +206,7 @@ __global__ __launch_bounds__(256) void conv3x3_lds_kernel(ConvArgs a) {
         const int t = k / a.cin, c = k - t * a.cin;
         const int yy = ay[i] + t / 3 - 1, xx = ax[i] + t % 3 - 1;
         if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-          v = *reinterpret_cast<const bf16x8*>(a.in + (((size_t)ab[i] * a.H + yy) * a.W + xx) * a.cin + c);
+          v = *reinterpret_cast<const bf16x8*>(conv_in(a.in, a.in2, a.c0, a.H, a.W, a.cin, ab[i], yy, xx, c));
       }
       ra[i] = v;
     }
@@ -258,6 +274,7 @@ __global__ __launch_bounds__(256) void conv3x3_lds_kernel(ConvArgs a) {
         if (m >= M) continue;
         float v = acc[i][j][r] + bias;
         if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.mask && !((float)a.mask[m * a.cout + co] > 0.0f)) v = 0.0f;
         if (a.out_f32) static_cast<float*>(a.out)[m * a.cout + co] = v;
         else static_cast<bf16*>(a.out)[m * a.cout + co] = (bf16)v;
       }
@@ -268,34 +285,58 @@ __global__ __launch_bounds__(256) void conv3x3_lds_kernel(ConvArgs a) {
 // Weight gradient dW[co][k] = sum_m dY[m][co] X[m + tap(k)][c(k)]: a GEMM
 // whose reduction runs over pixels m.  Block tile CO_T (co) x KT (k) with
 // KT = 256 / CO_T * 16 ... 4 waves, each a 16-row co strip x 64 k (4 MFMA
-// tiles).  The block walks its pixel range in chunks of 64.  Both operands
-// need 8 consecutive pixels of one channel per lane, so the chunk's dY and X
-// tiles are stored transposed in LDS ([channel][pixel]) and each MFMA
-// fragment is one 16-byte read.  Rows are 64 pixels (128 B) with the 16-byte
-// pixel groups XOR-swizzled by (row & 7) ^ ((row >> 3) & 7): the fragment
-// reads of 8 consecutive rows and the transposing 2-byte writes of 8 channel
-// groups both fall in distinct banks.  Result -> partial[blockIdx.z]
-// (pixel-range splits, added in a fixed order by reduce_partials_kernel).
+// tiles).  The block walks its pixel range in chunks of 64.  The chunk's dY
+// and X tiles are staged in LDS in their natural pixel-major layout
+// ([pixel][channel], one 16-byte write per 8 channels), and gfx950's
+// transposing LDS read (ds_read_b64_tr_b16) hands each lane the 8
+// consecutive pixels of one channel that an MFMA fragment needs, as two
+// 8-byte reads.  Within each 16-lane group that read gives lane i, value k,
+// the 16-bit element i % 4 of the 8 bytes addressed by lane 4k + i / 4; so
+// lane j addresses pixel row p0 + j / 4, channels c0 + 4 (j % 4) .. + 3.
+// Row strides of 80 / 48 / 144 bf16 (160 / 96 / 288 bytes, 32 or 96 mod 128)
+// put the 4 rows a 16-lane group reads in distinct banks.  Result ->
+// partial[blockIdx.z] (pixel-range splits, added in a fixed order by
+// reduce_partials_kernel).
 // ---------------------------------------------------------------------------
 struct WgradArgs {
-  const bf16* x;    // [B][H][W][cin]
+  const bf16* x;    // [B][H][W][cin] (with x2: the concatenation, as ConvArgs::in2)
   const bf16* dy;   // [B][H][W][cout]
   float* partial;   // [nsplit][cout][9 cin]
   int B, H, W, cin, cout;
   long long pix_per_split;  // multiple of 64
+  const bf16* x2;
+  int c0;
 };
 
-__device__ __forceinline__ int wg_swz(int row, int group) { return group ^ ((row & 7) ^ ((row >> 3) & 7)); }
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
+
+// An MFMA fragment of 8 consecutive pixels of one channel from a
+// pixel-major LDS tile: the transposing reads of pixel rows [p, p + 4) and
+// [p + 4, p + 8) (p, c as addressed by this lane, see above).
+__device__ __forceinline__ bf16x8 tr_fragment(const bf16* tile, int row, int col, int stride) {
+  const lds_i16x4* p0 = (const lds_i16x4*)((__attribute__((address_space(3))) char*)tile +
+                                           2 * (row * stride + col));
+  const lds_i16x4* p1 = (const lds_i16x4*)((__attribute__((address_space(3))) char*)tile +
+                                           2 * ((row + 4) * stride + col));
+  const i16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(const_cast<lds_i16x4*>(p0));
+  const i16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(const_cast<lds_i16x4*>(p1));
+  const i16x8 v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, v);
+}
 
 template <int CO_T>
 __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
   static_assert(CO_T == 64 || CO_T == 32, "co tile 64 or 32");
   constexpr int WK = 64 / CO_T * 1;       // waves along k: 1 (CO_T 64) or 2 (CO_T 32)
   constexpr int KT = 64 * WK;             // k tile
-  constexpr int DC = CO_T * 64 / 8 / 256; // dY chunks (8 channels of one pixel) per thread: 2 or 1
-  constexpr int XC = KT * 64 / 8 / 256;   // X chunks per thread: 2 or 4
-  __shared__ __attribute__((aligned(16))) bf16 s_dy[CO_T * 64];  // [co][m], swizzled groups
-  __shared__ __attribute__((aligned(16))) bf16 s_x[KT * 64];     // [k][m], swizzled groups
+  constexpr int SD = CO_T + 16, SX = KT + 16;  // LDS row strides (bf16)
+  constexpr int DG = CO_T / 8, XG = KT / 8;    // 16-byte groups per pixel row
+  constexpr int DC = 64 * DG / 256;       // dY groups per thread: 2 or 1
+  constexpr int XC = 64 * XG / 256;       // X groups per thread: 2 or 4
+  __shared__ __attribute__((aligned(16))) bf16 s_dy[64 * SD];  // [m][co]
+  __shared__ __attribute__((aligned(16))) bf16 s_x[64 * SX];   // [m][k]
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r16 = lane & 15, g = lane >> 4;
   const int wco = wv % (4 / WK), wk = wv / (4 / WK);  // this wave's co strip and k half
@@ -309,52 +350,79 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  for (long long m0 = m_begin; m0 < m_end; m0 += 64) {
-    // stage: chunk q -> pixel pm = q % 64, channel group cg = (q / 64) * 8
+  // Each thread stages the same 8 k values (one tap t, channels c .. c + 7)
+  // of X for XC pixels, pm = tid / XG + i (256 / XG), in every chunk; the
+  // chunk's first pixel (b, y, x0) advances by 64 pixels per chunk.
+  const int kx = k0 + (int)(threadIdx.x % XG) * 8;
+  const int tap = kx / a.cin, cx = kx - tap * a.cin;
+  const int oy = tap / 3 - 1, ox = tap % 3 - 1;
+  const bool kin = kx < K;
+  const int pmx = (int)threadIdx.x / XG;
+  const long long hw = (long long)a.H * a.W;
+  int cb = (int)(m_begin / hw);
+  int cy = (int)((m_begin - cb * hw) / a.W);
+  int cx0 = (int)(m_begin - cb * hw - (long long)cy * a.W);
+  // Operands of one chunk in registers: the next chunk's loads are issued
+  // before the current chunk's MFMAs and land in LDS after them.
+  bf16x8 rd[DC], rx[XC];
+  auto load = [&](long long m0) {
 #pragma unroll
     for (int i = 0; i < DC; ++i) {
-      const int q = threadIdx.x + 256 * i, pm = q & 63, cg = (q >> 6) * 8;
+      const int q = threadIdx.x + 256 * i, pm = q / DG, cg = (q % DG) * 8;
       const long long m = m0 + pm;
-      bf16x8 v = zero8();
-      if (m < m_end && co0 + cg < a.cout) v = *reinterpret_cast<const bf16x8*>(a.dy + m * a.cout + co0 + cg);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int row = cg + e;
-        s_dy[row * 64 + wg_swz(row, pm >> 3) * 8 + (pm & 7)] = v[e];
-      }
+      rd[i] = zero8();
+      if (m < m_end && co0 + cg < a.cout) rd[i] = *reinterpret_cast<const bf16x8*>(a.dy + m * a.cout + co0 + cg);
     }
 #pragma unroll
     for (int i = 0; i < XC; ++i) {
-      const int q = threadIdx.x + 256 * i, pm = q & 63, cg = (q >> 6) * 8;
-      const long long m = m0 + pm;
-      const int k = k0 + cg;
-      bf16x8 v = zero8();
-      if (m < m_end && k < K) {
-        const int t = k / a.cin, c = k - t * a.cin;
-        const int x = (int)(m % a.W);
-        const long long qq = m / a.W;
-        const int y = (int)(qq % a.H);
-        const int b = (int)(qq / a.H);
-        const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+      const int pm = pmx + i * (256 / XG);
+      rx[i] = zero8();
+      if (kin && m0 + pm < m_end) {
+        int x = cx0 + pm, y = cy, b = cb;
+        while (x >= a.W) {
+          x -= a.W;
+          if (++y == a.H) {
+            y = 0;
+            ++b;
+          }
+        }
+        const int yy = y + oy, xx = x + ox;
         if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-          v = *reinterpret_cast<const bf16x8*>(a.x + (((size_t)b * a.H + yy) * a.W + xx) * a.cin + c);
-      }
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int row = cg + e;
-        s_x[row * 64 + wg_swz(row, pm >> 3) * 8 + (pm & 7)] = v[e];
+          rx[i] = *reinterpret_cast<const bf16x8*>(conv_in(a.x, a.x2, a.c0, a.H, a.W, a.cin, b, yy, xx, cx));
       }
     }
+    // the chunk's first pixel moves on by 64
+    cx0 += 64;
+    while (cx0 >= a.W) {
+      cx0 -= a.W;
+      if (++cy == a.H) {
+        cy = 0;
+        ++cb;
+      }
+    }
+  };
+  if (m_begin < m_end) load(m_begin);
+  for (long long m0 = m_begin; m0 < m_end; m0 += 64) {
+    // stage: group q -> pixel pm = q / G, channels cg .. cg + 7, cg = (q % G) * 8
+#pragma unroll
+    for (int i = 0; i < DC; ++i) {
+      const int q = threadIdx.x + 256 * i, pm = q / DG, cg = (q % DG) * 8;
+      *reinterpret_cast<bf16x8*>(&s_dy[pm * SD + cg]) = rd[i];
+    }
+#pragma unroll
+    for (int i = 0; i < XC; ++i) {
+      const int pm = pmx + i * (256 / XG);
+      *reinterpret_cast<bf16x8*>(&s_x[pm * SX + (threadIdx.x % XG) * 8]) = rx[i];
+    }
     __syncthreads();
+    if (m0 + 64 < m_end) load(m0 + 64);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {  // two 32-pixel MFMA steps
-      const int grp = 4 * h + g;     // this lane's 8-pixel group
-      const int ra = 16 * wco + r16;
-      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&s_dy[ra * 64 + wg_swz(ra, grp) * 8]);
+      const int row = 32 * h + 8 * g + (r16 >> 2), col = 4 * (r16 & 3);
+      const bf16x8 av = tr_fragment(s_dy, row, 16 * wco + col, SD);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int rb = 64 * wk + 16 * j + r16;
-        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&s_x[rb * 64 + wg_swz(rb, grp) * 8]);
+        const bf16x8 bv = tr_fragment(s_x, row, 64 * wk + 16 * j + col, SX);
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
       }
     }
@@ -618,9 +686,10 @@ __global__ void to_bf16_kernel(const float* __restrict__ src, bf16* __restrict__
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-hipError_t nn_conv3x3(const void* in, const void* w, const float* bias, void* out, int B, int H, int W, int cin,
-                      int cout, int relu, int out_f32, hipStream_t s) {
-  ConvArgs a{static_cast<const bf16*>(in), static_cast<const bf16*>(w), bias, out, B, H, W, cin, cout, relu, out_f32};
+hipError_t nn_conv3x3(const void* in, const void* in2, int c0, const void* w, const float* bias, void* out,
+                      const void* mask, int B, int H, int W, int cin, int cout, int relu, int out_f32, hipStream_t s) {
+  ConvArgs a{static_cast<const bf16*>(in), static_cast<const bf16*>(w), bias, out, B, H, W, cin, cout, relu, out_f32,
+             static_cast<const bf16*>(mask), static_cast<const bf16*>(in2), c0};
   const long long M = (long long)B * H * W;
   if (cin % 8 != 0) return hipErrorInvalidValue;
   if (cout % 128 == 0) {
@@ -641,13 +710,14 @@ long long nn_wgrad_splits(long long M) {
   return n < 1 ? 1 : n;
 }
 
-hipError_t nn_wgrad(const void* x, const void* dy, float* dw, float* partial, int B, int H, int W, int cin, int cout,
-                    hipStream_t s) {
+hipError_t nn_wgrad(const void* x, const void* x2, int c0, const void* dy, float* dw, float* partial, int B, int H,
+                    int W, int cin, int cout, hipStream_t s) {
   const long long M = (long long)B * H * W;
   const long long ns = nn_wgrad_splits(M);
   const long long per = ((M + ns - 1) / ns + 63) / 64 * 64;
   const int K = 9 * cin;
-  WgradArgs a{static_cast<const bf16*>(x), static_cast<const bf16*>(dy), partial, B, H, W, cin, cout, per};
+  WgradArgs a{static_cast<const bf16*>(x), static_cast<const bf16*>(dy), partial, B, H, W, cin, cout, per,
+              static_cast<const bf16*>(x2), c0};
   if (cout <= 32) wgrad_kernel<32><<<dim3(1, (K + 127) / 128, (unsigned)ns), 256, 0, s>>>(a);
   else wgrad_kernel<64><<<dim3((cout + 63) / 64, (K + 63) / 64, (unsigned)ns), 256, 0, s>>>(a);
   const long long n = (long long)cout * K;

@@ -118,12 +118,18 @@ class UNet:
         out["out"] = (self.head_w.cpu().numpy().reshape(1, 1, 1, self.c0), self.head_b.cpu().numpy())
         return out
 
-    def _conv(self, c, inp, out, B, H, W, relu=True, w=None, bias=True, out_f32=False, cin=None, cout=None):
+    def _conv(self, c, inp, out, B, H, W, relu=True, w=None, bias=True, out_f32=False, cin=None, cout=None,
+              skip=None, mask=None):
+        """3x3 convolution.  inp may be (low, skip_tensor): the decoder's
+        concatenation [upsample2(low), skip] read in place; mask (an
+        activation) zeroes the output where it is not positive."""
         L = _lib.lib()
         w = c.w16 if w is None else w
         b = _vp(c.b32) if bias else None
-        self._chk(L.sgdct_nn_conv3x3(_vp(inp), _vp(w), b, _vp(out), B, H, W, cin or c.cin, cout or c.cout,
-                                     int(relu), int(out_f32), self._s()), "conv3x3 " + c.name)
+        low, in2, c0 = (inp[0], inp[1], inp[0].shape[3]) if isinstance(inp, tuple) else (inp, None, 0)
+        self._chk(L.sgdct_nn_conv3x3_ex(_vp(low), _vp(in2) if in2 is not None else None, c0, _vp(w), b, _vp(out),
+                                        _vp(mask) if mask is not None else None, B, H, W, cin or c.cin,
+                                        cout or c.cout, int(relu), int(out_f32), self._s()), "conv3x3 " + c.name)
 
     # -- forward -------------------------------------------------------------
     def forward(self, x):
@@ -160,10 +166,9 @@ class UNet:
         for l in range(self.levels - 2, -1, -1):
             h, w = H >> l, W >> l
             ca, cb = self.convs["dec%d_a" % l], self.convs["dec%d_b" % l]
-            cu = ca.cin - enc[l].shape[3]
-            src = self._get("dec%d_in" % l, (B, h, w, ca.cin), bf)
-            self._chk(L.sgdct_nn_upcat(_vp(d), _vp(enc[l]), _vp(src), B, h, w, cu, enc[l].shape[3], self._s()),
-                      "upcat")
+            # the concatenation [upsample2(d), enc[l]] is read in place by the
+            # convolution and by its weight gradient
+            src = (d, enc[l])
             a = self._get("dec%d_a" % l, (B, h, w, ca.cout), bf)
             self._conv(ca, src, a, B, h, w)
             d = self._get("dec%d_b" % l, (B, h, w, cb.cout), bf)
@@ -187,19 +192,24 @@ class UNet:
         return ws
 
     def _conv_grads(self, c, x_in, gy, B, h, w):
-        """dW, db of conv c from its input activation and bf16 output gradient."""
+        """dW, db of conv c from its input activation (or (low, skip) for a
+        decoder's concatenated input) and bf16 output gradient."""
         L = _lib.lib()
         M = B * h * w
         need = max(L.sgdct_nn_wgrad_workspace(B, h, w, c.cin, c.cout), L.sgdct_nn_colsum_workspace(M, c.cout))
         ws = self._workspace(need)
-        self._chk(L.sgdct_nn_wgrad(_vp(x_in), _vp(gy), _vp(c.gw), _vp(ws), ws.numel(), B, h, w, c.cin, c.cout,
-                                   self._s()), "wgrad " + c.name)
+        low, x2, c0 = (x_in[0], x_in[1], x_in[0].shape[3]) if isinstance(x_in, tuple) else (x_in, None, 0)
+        self._chk(L.sgdct_nn_wgrad_ex(_vp(low), _vp(x2) if x2 is not None else None, c0, _vp(gy), _vp(c.gw),
+                                      _vp(ws), ws.numel(), B, h, w, c.cin, c.cout, self._s()), "wgrad " + c.name)
         self._chk(L.sgdct_nn_colsum(_vp(gy), _vp(c.gb), _vp(ws), ws.numel(), M, c.cout, self._s()), "colsum")
 
-    def _input_grad(self, c, gy, out, B, h, w):
+    def _input_grad(self, c, gy, out, B, h, w, mask=None):
+        """Gradient of conv c's input: FP32, or with mask (the input's own
+        ReLU activation) the bf16 output gradient of the layer before."""
         L = _lib.lib()
         self._chk(L.sgdct_nn_flip_weights(_vp(c.w16), _vp(c.wt16), c.cin, c.cout, self._s()), "flip")
-        self._conv(c, gy, out, B, h, w, relu=False, w=c.wt16, bias=False, out_f32=True, cin=c.cout, cout=c.cin)
+        self._conv(c, gy, out, B, h, w, relu=False, w=c.wt16, bias=False, out_f32=mask is None, cin=c.cout,
+                   cout=c.cin, mask=mask)
 
     def backward(self, target):
         """Gradients of mean((y - target)^2) into every parameter's .gw/.gb
@@ -229,11 +239,8 @@ class UNet:
             self._chk(L.sgdct_nn_relu_mask(_vp(gdec[l]), _vp(t["dec%d_b" % l]), _vp(gy), gy.numel(), self._s()),
                       "relu_mask")
             self._conv_grads(cb, t["dec%d_a" % l], gy, B, h, w)
-            ga = self._get("ga%d" % l, (B, h, w, cb.cin), f32)
-            self._input_grad(cb, gy, ga, B, h, w)
             gy2 = self._get("gy%d_a" % l, (B, h, w, ca.cout), bf)
-            self._chk(L.sgdct_nn_relu_mask(_vp(ga), _vp(t["dec%d_a" % l]), _vp(gy2), gy2.numel(), self._s()),
-                      "relu_mask")
+            self._input_grad(cb, gy, gy2, B, h, w, mask=t["dec%d_a" % l])
             self._conv_grads(ca, t["dec%d_in" % l], gy2, B, h, w)
             gin = self._get("gin%d" % l, (B, h, w, ca.cin), f32)
             self._input_grad(ca, gy2, gin, B, h, w)
@@ -252,11 +259,8 @@ class UNet:
             self._chk(L.sgdct_nn_relu_mask(_vp(genc[l]), _vp(t["enc%d_b" % l]), _vp(gy), gy.numel(), self._s()),
                       "relu_mask")
             self._conv_grads(cb, t["enc%d_a" % l], gy, B, h, w)
-            ga = self._get("gae%d" % l, (B, h, w, cb.cin), f32)
-            self._input_grad(cb, gy, ga, B, h, w)
             gy2 = self._get("gye%d_a" % l, (B, h, w, ca.cout), bf)
-            self._chk(L.sgdct_nn_relu_mask(_vp(ga), _vp(t["enc%d_a" % l]), _vp(gy2), gy2.numel(), self._s()),
-                      "relu_mask")
+            self._input_grad(cb, gy, gy2, B, h, w, mask=t["enc%d_a" % l])
             self._conv_grads(ca, t["enc%d_in" % l], gy2, B, h, w)
             if l > 0:
                 gin = self._get("gine%d" % l, (B, h, w, ca.cin), f32)
