@@ -281,28 +281,37 @@ long long sgdct_nn_wgrad_workspace(int B, int H, int W, int cin, int cout);
 int sgdct_nn_wgrad(const void *x, const void *dy, float *dw, float *workspace, long long workspace_floats, int B,
                    int H, int W, int cin, int cout, void *stream);
 /* sgdct_nn_wgrad of the concatenated input [upsample2(x), x2] (as
- * sgdct_nn_conv3x3_ex; x2 = NULL is sgdct_nn_wgrad). */
-int sgdct_nn_wgrad_ex(const void *x, const void *x2, int c0, const void *dy, float *dw, float *workspace,
+ * sgdct_nn_conv3x3_ex; x2 = NULL is sgdct_nn_wgrad), and with db != NULL also
+ * the bias gradient db[co] = sum over pixels of dy[m][co] (FP32 sums). */
+int sgdct_nn_wgrad_ex(const void *x, const void *x2, int c0, const void *dy, float *dw, float *db, float *workspace,
                       long long workspace_floats, int B, int H, int W, int cin, int cout, void *stream);
 long long sgdct_nn_colsum_workspace(long long M, int C);
 int sgdct_nn_colsum(const void *a, float *out, float *workspace, long long workspace_floats, long long M, int C,
                     void *stream);
 int sgdct_nn_pad_input(const float *x, void *out8, long long npix, void *stream);
-/* dst (B, H, W, C) = 2x2 average of src (B, 2H, 2W, C); back: gsrc += dv / 4 */
+/* The gradients below that end in a ReLU activation's mask can be written
+ * masked and in bf16 directly: given out (bf16) and mask (the activation,
+ * bf16), out = bf16(value * (mask > 0)) replaces the FP32 result. */
+/* dst (B, H, W, C) = 2x2 average of src (B, 2H, 2W, C); back: gsrc += dv / 4
+ * (with out: out = masked gsrc + dv / 4, gsrc unchanged) */
 int sgdct_nn_pool2(const void *src, void *dst, int B, int H, int W, int C, void *stream);
-int sgdct_nn_pool2_back(const float *dv, float *gsrc, int B, int H, int W, int C, void *stream);
-/* dst (B, H, W, C0 + C1) = [up2(low (B, H/2, W/2, C0)), skip (B, H, W, C1)];
- * back: glow = sum over each 2x2 block of dv[..., :C0], gskip += dv[..., C0:] */
-int sgdct_nn_upcat(const void *low, const void *skip, void *dst, int B, int H, int W, int C0, int C1, void *stream);
-int sgdct_nn_upcat_back(const float *dv, float *glow, float *gskip, int B, int H, int W, int C0, int C1,
+int sgdct_nn_pool2_back(const float *dv, float *gsrc, void *out, const void *mask, int B, int H, int W, int C,
                         void *stream);
+/* dst (B, H, W, C0 + C1) = [up2(low (B, H/2, W/2, C0)), skip (B, H, W, C1)];
+ * back: glow = sum over each 2x2 block of dv[..., :C0] (with glow_out: masked
+ * into glow_out), gskip = dv[..., C0:] (+= with skip_accumulate) */
+int sgdct_nn_upcat(const void *low, const void *skip, void *dst, int B, int H, int W, int C0, int C1, void *stream);
+int sgdct_nn_upcat_back(const float *dv, float *glow, void *glow_out, const void *glow_mask, float *gskip,
+                        int skip_accumulate, int B, int H, int W, int C0, int C1, void *stream);
 /* y = x + d0 . w + b (1x1 output head); back: dy = 2 (y - target) / M,
- * gd0 = dy w, gw[0:C] = sum dy d0, gw[C] = sum dy */
+ * gd0 = dy w (with gd0_out: masked into gd0_out), gw[0:C] = sum dy d0,
+ * gw[C] = sum dy */
 int sgdct_nn_head(const void *d0, const float *w, const float *b, const float *x, float *y, long long M, int C,
                   void *stream);
 long long sgdct_nn_head_workspace(long long M, int C);
 int sgdct_nn_head_back(const float *y, const float *target, const float *w, const void *d0, float *dy, float *gd0,
-                       float *gw, float *workspace, long long workspace_floats, long long M, int C, void *stream);
+                       void *gd0_out, const void *mask, float *gw, float *workspace, long long workspace_floats,
+                       long long M, int C, void *stream);
 /* out = bf16(g * (act > 0)) */
 int sgdct_nn_relu_mask(const float *g, const void *act, void *out, long long n, void *stream);
 /* wt[c][t][o] = w[o][8 - t][c] */

@@ -89,17 +89,17 @@ def lib():
         "sgdct_nn_wgrad_workspace": [_i] * 5,
         "sgdct_nn_wgrad": [_v, _v, _v, _v, _ll] + [_i] * 5 + [_v],
         "sgdct_nn_conv3x3_ex": [_v, _v, _i, _v, _v, _v, _v] + [_i] * 7 + [_v],
-        "sgdct_nn_wgrad_ex": [_v, _v, _i, _v, _v, _v, _ll] + [_i] * 5 + [_v],
+        "sgdct_nn_wgrad_ex": [_v, _v, _i, _v, _v, _v, _v, _ll] + [_i] * 5 + [_v],
         "sgdct_nn_colsum_workspace": [_ll, _i],
         "sgdct_nn_colsum": [_v, _v, _v, _ll, _ll, _i, _v],
         "sgdct_nn_pad_input": [_v, _v, _ll, _v],
         "sgdct_nn_pool2": [_v, _v] + [_i] * 4 + [_v],
-        "sgdct_nn_pool2_back": [_v, _v] + [_i] * 4 + [_v],
+        "sgdct_nn_pool2_back": [_v, _v, _v, _v] + [_i] * 4 + [_v],
         "sgdct_nn_upcat": [_v, _v, _v] + [_i] * 5 + [_v],
-        "sgdct_nn_upcat_back": [_v, _v, _v] + [_i] * 5 + [_v],
+        "sgdct_nn_upcat_back": [_v, _v, _v, _v, _v] + [_i] * 6 + [_v],
         "sgdct_nn_head": [_v] * 5 + [_ll, _i, _v],
         "sgdct_nn_head_workspace": [_ll, _i],
-        "sgdct_nn_head_back": [_v] * 8 + [_ll, _ll, _i, _v],
+        "sgdct_nn_head_back": [_v] * 10 + [_ll, _ll, _i, _v],
         "sgdct_nn_relu_mask": [_v, _v, _v, _ll, _v],
         "sgdct_nn_flip_weights": [_v, _v, _i, _i, _v],
         "sgdct_nn_adam": [_v] * 5 + [_ll, _f, _f, _f, _f, _i, _v],

@@ -54,20 +54,22 @@ hipError_t launch_batch_forward(const BatchPlan& P, int blocks, const float* img
 hipError_t nn_conv3x3(const void* in, const void* in2, int c0, const void* w, const float* bias, void* out,
                       const void* mask, int B, int H, int W, int cin, int cout, int relu, int out_f32, hipStream_t s);
 long long nn_wgrad_splits(long long M);
-hipError_t nn_wgrad(const void* x, const void* x2, int c0, const void* dy, float* dw, float* partial, int B, int H,
-                    int W, int cin, int cout, hipStream_t s);
+hipError_t nn_wgrad(const void* x, const void* x2, int c0, const void* dy, float* dw, float* db, float* partial, int B,
+                    int H, int W, int cin, int cout, hipStream_t s);
 hipError_t nn_colsum_bf16(const void* a, float* out, float* partial, long long M, int C, hipStream_t s);
 hipError_t nn_pad_input(const float* x, void* out8, long long n, hipStream_t s);
 hipError_t nn_pool2(const void* src, void* dst, int B, int H, int W, int C, hipStream_t s);
-hipError_t nn_pool2_back(const float* dv, float* gsrc, int B, int H, int W, int C, hipStream_t s);
+hipError_t nn_pool2_back(const float* dv, float* gsrc, void* out, const void* mask, int B, int H, int W, int C,
+                         hipStream_t s);
 hipError_t nn_upcat(const void* low, const void* skip, void* dst, int B, int H, int W, int C0, int C1,
                     hipStream_t s);
-hipError_t nn_upcat_back(const float* dv, float* glow, float* gskip, int B, int H, int W, int C0, int C1,
-                         hipStream_t s);
+hipError_t nn_upcat_back(const float* dv, float* glow, void* glow_out, const void* glow_mask, float* gskip,
+                         int skip_accumulate, int B, int H, int W, int C0, int C1, hipStream_t s);
 hipError_t nn_head(const void* d0, const float* w, const float* b, const float* x, float* y, long long M, int C,
                    hipStream_t s);
 hipError_t nn_head_back(const float* y, const float* target, const float* w, const void* d0, float* dy, float* gd0,
-                        float* gw, float* partial, long long M, int C, hipStream_t s);
+                        void* gd0_out, const void* mask, float* gw, float* partial, long long M, int C,
+                        hipStream_t s);
 hipError_t nn_relu_mask(const float* g, const void* act, void* out, long long n, hipStream_t s);
 hipError_t nn_flip_weights(const void* w, void* wt, int cin, int cout, hipStream_t s);
 hipError_t nn_adam(float* w32, void* w16, const float* g, float* m, float* v, long long n, float lr, float b1,
@@ -1241,15 +1243,15 @@ int sgdct_nn_conv3x3_ex(const void* in, const void* in2, int c0, const void* w, 
 
 long long sgdct_nn_wgrad_workspace(int B, int H, int W, int cin, int cout) {
   if (!dims_ok(B, H, W) || cin <= 0 || cout <= 0) return fail(SGDCT_EINVAL, "bad wgrad shape");
-  return nn_wgrad_splits((long long)B * H * W) * cout * 9LL * cin;
+  return nn_wgrad_splits((long long)B * H * W) * cout * (9LL * cin + 1);
 }
 
 int sgdct_nn_wgrad(const void* x, const void* dy, float* dw, float* workspace, long long workspace_floats, int B,
                    int H, int W, int cin, int cout, void* stream) {
-  return sgdct_nn_wgrad_ex(x, nullptr, 0, dy, dw, workspace, workspace_floats, B, H, W, cin, cout, stream);
+  return sgdct_nn_wgrad_ex(x, nullptr, 0, dy, dw, nullptr, workspace, workspace_floats, B, H, W, cin, cout, stream);
 }
 
-int sgdct_nn_wgrad_ex(const void* x, const void* x2, int c0, const void* dy, float* dw, float* workspace,
+int sgdct_nn_wgrad_ex(const void* x, const void* x2, int c0, const void* dy, float* dw, float* db, float* workspace,
                       long long workspace_floats, int B, int H, int W, int cin, int cout, void* stream) {
   if (!x || !dy || !dw || !workspace || !dims_ok(B, H, W) || cin <= 0 || cin % 8 || cout <= 0 || cout % 8)
     return fail(SGDCT_EINVAL, "bad wgrad arguments (cin and cout must be multiples of 8)");
@@ -1257,7 +1259,7 @@ int sgdct_nn_wgrad_ex(const void* x, const void* x2, int c0, const void* dy, flo
     return fail(SGDCT_EINVAL, "bad wgrad concatenation (0 < c0 < cin, c0 % 8 == 0, even H and W)");
   if (workspace_floats < sgdct_nn_wgrad_workspace(B, H, W, cin, cout))
     return fail(SGDCT_EINVAL, "wgrad workspace too small");
-  return nn_ret(nn_wgrad(x, x2, c0, dy, dw, workspace, B, H, W, cin, cout, (hipStream_t)stream), "wgrad");
+  return nn_ret(nn_wgrad(x, x2, c0, dy, dw, db, workspace, B, H, W, cin, cout, (hipStream_t)stream), "wgrad");
 }
 
 long long sgdct_nn_colsum_workspace(long long M, int C) {
@@ -1282,9 +1284,11 @@ int sgdct_nn_pool2(const void* src, void* dst, int B, int H, int W, int C, void*
   return nn_ret(nn_pool2(src, dst, B, H, W, C, (hipStream_t)stream), "pool2");
 }
 
-int sgdct_nn_pool2_back(const float* dv, float* gsrc, int B, int H, int W, int C, void* stream) {
-  if (!dv || !gsrc || !dims_ok(B, H, W) || C <= 0) return fail(SGDCT_EINVAL, "bad pool2_back arguments");
-  return nn_ret(nn_pool2_back(dv, gsrc, B, H, W, C, (hipStream_t)stream), "pool2_back");
+int sgdct_nn_pool2_back(const float* dv, float* gsrc, void* out, const void* mask, int B, int H, int W, int C,
+                        void* stream) {
+  if (!dv || !gsrc || !dims_ok(B, H, W) || C <= 0 || (!out) != (!mask))
+    return fail(SGDCT_EINVAL, "bad pool2_back arguments");
+  return nn_ret(nn_pool2_back(dv, gsrc, out, mask, B, H, W, C, (hipStream_t)stream), "pool2_back");
 }
 
 int sgdct_nn_upcat(const void* low, const void* skip, void* dst, int B, int H, int W, int C0, int C1, void* stream) {
@@ -1293,11 +1297,14 @@ int sgdct_nn_upcat(const void* low, const void* skip, void* dst, int B, int H, i
   return nn_ret(nn_upcat(low, skip, dst, B, H, W, C0, C1, (hipStream_t)stream), "upcat");
 }
 
-int sgdct_nn_upcat_back(const float* dv, float* glow, float* gskip, int B, int H, int W, int C0, int C1,
-                        void* stream) {
-  if (!dv || !glow || !gskip || !dims_ok(B, H, W) || H % 2 || W % 2 || C0 <= 0 || C1 <= 0)
+int sgdct_nn_upcat_back(const float* dv, float* glow, void* glow_out, const void* glow_mask, float* gskip,
+                        int skip_accumulate, int B, int H, int W, int C0, int C1, void* stream) {
+  if (!dv || !(glow_out ? glow_mask != nullptr : glow != nullptr) || !gskip || !dims_ok(B, H, W) || H % 2 || W % 2 ||
+      C0 <= 0 || C1 <= 0)
     return fail(SGDCT_EINVAL, "bad upcat_back arguments");
-  return nn_ret(nn_upcat_back(dv, glow, gskip, B, H, W, C0, C1, (hipStream_t)stream), "upcat_back");
+  return nn_ret(nn_upcat_back(dv, glow, glow_out, glow_mask, gskip, skip_accumulate, B, H, W, C0, C1,
+                              (hipStream_t)stream),
+                "upcat_back");
 }
 
 int sgdct_nn_head(const void* d0, const float* w, const float* b, const float* x, float* y, long long M, int C,
@@ -1312,11 +1319,14 @@ long long sgdct_nn_head_workspace(long long M, int C) {
 }
 
 int sgdct_nn_head_back(const float* y, const float* target, const float* w, const void* d0, float* dy, float* gd0,
-                       float* gw, float* workspace, long long workspace_floats, long long M, int C, void* stream) {
-  if (!y || !target || !w || !d0 || !dy || !gd0 || !gw || !workspace || M <= 0 || C <= 0 || C > 255)
+                       void* gd0_out, const void* mask, float* gw, float* workspace, long long workspace_floats,
+                       long long M, int C, void* stream) {
+  if (!y || !target || !w || !d0 || !dy || !(gd0_out ? mask != nullptr : gd0 != nullptr) || !gw || !workspace ||
+      M <= 0 || C <= 0 || C > 255)
     return fail(SGDCT_EINVAL, "bad head_back arguments");
   if (workspace_floats < sgdct_nn_head_workspace(M, C)) return fail(SGDCT_EINVAL, "head workspace too small");
-  return nn_ret(nn_head_back(y, target, w, d0, dy, gd0, gw, workspace, M, C, (hipStream_t)stream), "head_back");
+  return nn_ret(nn_head_back(y, target, w, d0, dy, gd0, gd0_out, mask, gw, workspace, M, C, (hipStream_t)stream),
+                "head_back");
 }
 
 int sgdct_nn_relu_mask(const float* g, const void* act, void* out, long long n, void* stream) {

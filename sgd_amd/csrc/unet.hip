@@ -306,6 +306,7 @@ struct WgradArgs {
   long long pix_per_split;  // multiple of 64
   const bf16* x2;
   int c0;
+  float* bias_partial;      // [nsplit][cout] column sums of dy, or null
 };
 
 typedef short i16x4 __attribute__((ext_vector_type(4)));
@@ -401,6 +402,10 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
       }
     }
   };
+  // Bias gradient (the blocks of the first k tile): each thread sums the dY
+  // values it stages, channels (tid % DG) * 8 .. + 7, in FP32.
+  const bool bias = a.bias_partial && blockIdx.y == 0;
+  float bs[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (m_begin < m_end) load(m_begin);
   for (long long m0 = m_begin; m0 < m_end; m0 += 64) {
     // stage: group q -> pixel pm = q / G, channels cg .. cg + 7, cg = (q % G) * 8
@@ -408,6 +413,11 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
     for (int i = 0; i < DC; ++i) {
       const int q = threadIdx.x + 256 * i, pm = q / DG, cg = (q % DG) * 8;
       *reinterpret_cast<bf16x8*>(&s_dy[pm * SD + cg]) = rd[i];
+      if (bias) {
+        const i16x8 u = __builtin_bit_cast(i16x8, rd[i]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bs[e] += __uint_as_float((unsigned)(unsigned short)u[e] << 16);
+      }
     }
 #pragma unroll
     for (int i = 0; i < XC; ++i) {
@@ -436,6 +446,21 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
       const int co = co0 + 16 * wco + 4 * g + r, k = k0 + 64 * wk + 16 * j + r16;
       if (co < a.cout && k < K) out[(size_t)co * K + k] = acc[j][r];
     }
+  if (bias) {
+    // the 256 / DG threads of a channel group, added in thread order through
+    // the (now idle) X tile
+    float* red = reinterpret_cast<float*>(s_x);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) red[threadIdx.x * 8 + e] = bs[e];
+    __syncthreads();
+    if ((int)threadIdx.x < CO_T) {
+      const int grp = threadIdx.x / 8, e = threadIdx.x % 8;
+      float t = 0.0f;
+      for (int j = grp; j < 256; j += DG) t += red[j * 8 + e];
+      if (co0 + (int)threadIdx.x < a.cout)
+        a.bias_partial[(size_t)blockIdx.z * a.cout + co0 + threadIdx.x] = t;
+    }
+  }
 }
 
 // out[i] = sum_s partial[s][i] (fixed order)
@@ -520,7 +545,10 @@ __global__ void pool2_kernel(const bf16* __restrict__ src, bf16* __restrict__ ds
 }
 
 // g_src[b][2y+i][2x+j][c] += dv[b][y][x][c] / 4   (one thread per source element)
-__global__ void pool2_back_kernel(const float* __restrict__ dv, float* __restrict__ gsrc, int B, int H, int W, int C) {
+// gsrc += dv / 4 over each 2x2 block; with out (bf16) the sum, masked by
+// mask > 0, goes to out instead (the last contribution to that gradient)
+__global__ void pool2_back_kernel(const float* __restrict__ dv, float* __restrict__ gsrc, bf16* __restrict__ out,
+                                  const bf16* __restrict__ mask, int B, int H, int W, int C) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)B * 2 * H * 2 * W * C;
   if (i >= n) return;
@@ -530,7 +558,9 @@ __global__ void pool2_back_kernel(const float* __restrict__ dv, float* __restric
   q /= 2 * W;
   const int y = (int)(q % (2 * H));
   const int b = (int)(q / (2 * H));
-  gsrc[i] += 0.25f * dv[(((size_t)b * H + y / 2) * W + x / 2) * C + c];
+  const float v = gsrc[i] + 0.25f * dv[(((size_t)b * H + y / 2) * W + x / 2) * C + c];
+  if (out) out[i] = (bf16)((float)mask[i] > 0.0f ? v : 0.0f);
+  else gsrc[i] = v;
 }
 
 // dst[b][y][x][0:C0] = low[b][y/2][x/2][:], dst[..][C0:C0+C1] = skip[b][y][x][:]
@@ -551,8 +581,10 @@ __global__ void upcat_kernel(const bf16* __restrict__ low, const bf16* __restric
 }
 
 // g_low[b][y'][x'][c] = sum of the four children (c < C0); g_skip += rest
-__global__ void upcat_back_low_kernel(const float* __restrict__ dv, float* __restrict__ glow, int B, int H, int W,
-                                      int C0, int C1) {
+// glow = sum over each 2x2 block of dv[..., :C0]; with out (bf16), masked by
+// mask > 0, into out instead
+__global__ void upcat_back_low_kernel(const float* __restrict__ dv, float* __restrict__ glow, bf16* __restrict__ out,
+                                      const bf16* __restrict__ mask, int B, int H, int W, int C0, int C1) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int Hl = H / 2, Wl = W / 2, C = C0 + C1;
   const long long n = (long long)B * Hl * Wl * C0;
@@ -565,17 +597,21 @@ __global__ void upcat_back_low_kernel(const float* __restrict__ dv, float* __res
   const int b = (int)(q / Hl);
   const size_t o = (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
   const size_t row = (size_t)W * C;
-  glow[i] = dv[o] + dv[o + C] + dv[o + row] + dv[o + row + C];
+  const float v = dv[o] + dv[o + C] + dv[o + row] + dv[o + row + C];
+  if (out) out[i] = (bf16)((float)mask[i] > 0.0f ? v : 0.0f);
+  else glow[i] = v;
 }
 
+// gskip (+)= dv[..., C0:]
 __global__ void upcat_back_skip_kernel(const float* __restrict__ dv, float* __restrict__ gskip, int B, int H, int W,
-                                       int C0, int C1) {
+                                       int C0, int C1, int accumulate) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)B * H * W * C1;
   if (i >= n) return;
   const int c = (int)(i % C1);
   const long long p = i / C1;
-  gskip[i] += dv[p * (C0 + C1) + C0 + c];
+  const float v = dv[p * (C0 + C1) + C0 + c];
+  gskip[i] = accumulate ? gskip[i] + v : v;
 }
 
 // y[m] = x[m] + sum_c d0[m][c] w[c] + b
@@ -584,19 +620,36 @@ __global__ void head_kernel(const bf16* __restrict__ d0, const float* __restrict
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   float s = b[0];
-  for (int c = 0; c < C; ++c) s = __builtin_fmaf((float)d0[m * C + c], w[c], s);
+  if (C % 8 == 0) {
+    // 16-byte loads of 8 channels
+    for (int c = 0; c < C; c += 8) {
+      const i16x8 u = __builtin_bit_cast(i16x8, *reinterpret_cast<const bf16x8*>(d0 + m * C + c));
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s = __builtin_fmaf(__uint_as_float((unsigned)(unsigned short)u[e] << 16), w[c + e], s);
+    }
+  } else {
+    for (int c = 0; c < C; ++c) s = __builtin_fmaf((float)d0[m * C + c], w[c], s);
+  }
   y[m] = x[m] + s;
 }
 
-// dy = 2 (y - target) / n; g_d0[m][c] = dy[m] w[c]
+// dy = 2 (y - target) / n
+// gd0 = dy w (FP32), or with out: bf16(dy w * (mask > 0)) into out.  One
+// thread per (pixel, channel), so the gradient stores are coalesced; the
+// channel-0 thread of a pixel also stores dy.
 __global__ void head_back_kernel(const float* __restrict__ y, const float* __restrict__ target,
                                  const float* __restrict__ w, float* __restrict__ dy, float* __restrict__ gd0,
-                                 long long M, int C, float scale) {
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M) return;
+                                 bf16* __restrict__ out, const bf16* __restrict__ mask, long long M, int C,
+                                 float scale) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * C) return;
+  const long long m = i / C;
+  const int c = (int)(i - m * C);
   const float d = scale * (y[m] - target[m]);
-  dy[m] = d;
-  for (int c = 0; c < C; ++c) gd0[m * C + c] = d * w[c];
+  if (c == 0) dy[m] = d;
+  const float v = d * w[c];
+  if (out) out[i] = (bf16)((float)mask[i] > 0.0f ? v : 0.0f);
+  else gd0[i] = v;
 }
 
 // head weight gradient partials: part[s][c] = sum_m dy[m] d0[m][c], part[s][C] = sum_m dy[m]
@@ -710,18 +763,22 @@ long long nn_wgrad_splits(long long M) {
   return n < 1 ? 1 : n;
 }
 
-hipError_t nn_wgrad(const void* x, const void* x2, int c0, const void* dy, float* dw, float* partial, int B, int H,
-                    int W, int cin, int cout, hipStream_t s) {
+// partial holds nsplit * cout * (9 cin + 1) floats: the weight partials,
+// then (with db) the bias partials.
+hipError_t nn_wgrad(const void* x, const void* x2, int c0, const void* dy, float* dw, float* db, float* partial, int B,
+                    int H, int W, int cin, int cout, hipStream_t s) {
   const long long M = (long long)B * H * W;
   const long long ns = nn_wgrad_splits(M);
   const long long per = ((M + ns - 1) / ns + 63) / 64 * 64;
   const int K = 9 * cin;
+  float* bpart = db ? partial + ns * cout * K : nullptr;
   WgradArgs a{static_cast<const bf16*>(x), static_cast<const bf16*>(dy), partial, B, H, W, cin, cout, per,
-              static_cast<const bf16*>(x2), c0};
+              static_cast<const bf16*>(x2), c0, bpart};
   if (cout <= 32) wgrad_kernel<32><<<dim3(1, (K + 127) / 128, (unsigned)ns), 256, 0, s>>>(a);
   else wgrad_kernel<64><<<dim3((cout + 63) / 64, (K + 63) / 64, (unsigned)ns), 256, 0, s>>>(a);
   const long long n = (long long)cout * K;
   reduce_partials_kernel<<<blocks_for(n), 256, 0, s>>>(partial, dw, n, (int)ns);
+  if (db) reduce_partials_kernel<<<blocks_for(cout), 256, 0, s>>>(bpart, db, cout, (int)ns);
   return hipGetLastError();
 }
 
@@ -743,8 +800,10 @@ hipError_t nn_pool2(const void* src, void* dst, int B, int H, int W, int C, hipS
   return hipGetLastError();
 }
 
-hipError_t nn_pool2_back(const float* dv, float* gsrc, int B, int H, int W, int C, hipStream_t s) {
-  pool2_back_kernel<<<blocks_for((long long)B * 4 * H * W * C), 256, 0, s>>>(dv, gsrc, B, H, W, C);
+hipError_t nn_pool2_back(const float* dv, float* gsrc, void* out, const void* mask, int B, int H, int W, int C,
+                         hipStream_t s) {
+  pool2_back_kernel<<<blocks_for((long long)B * 4 * H * W * C), 256, 0, s>>>(
+      dv, gsrc, static_cast<bf16*>(out), static_cast<const bf16*>(mask), B, H, W, C);
   return hipGetLastError();
 }
 
@@ -755,10 +814,12 @@ hipError_t nn_upcat(const void* low, const void* skip, void* dst, int B, int H, 
   return hipGetLastError();
 }
 
-hipError_t nn_upcat_back(const float* dv, float* glow, float* gskip, int B, int H, int W, int C0, int C1,
-                         hipStream_t s) {
-  upcat_back_low_kernel<<<blocks_for((long long)B * (H / 2) * (W / 2) * C0), 256, 0, s>>>(dv, glow, B, H, W, C0, C1);
-  upcat_back_skip_kernel<<<blocks_for((long long)B * H * W * C1), 256, 0, s>>>(dv, gskip, B, H, W, C0, C1);
+hipError_t nn_upcat_back(const float* dv, float* glow, void* glow_out, const void* glow_mask, float* gskip,
+                         int skip_accumulate, int B, int H, int W, int C0, int C1, hipStream_t s) {
+  upcat_back_low_kernel<<<blocks_for((long long)B * (H / 2) * (W / 2) * C0), 256, 0, s>>>(
+      dv, glow, static_cast<bf16*>(glow_out), static_cast<const bf16*>(glow_mask), B, H, W, C0, C1);
+  upcat_back_skip_kernel<<<blocks_for((long long)B * H * W * C1), 256, 0, s>>>(dv, gskip, B, H, W, C0, C1,
+                                                                               skip_accumulate);
   return hipGetLastError();
 }
 
@@ -769,8 +830,10 @@ hipError_t nn_head(const void* d0, const float* w, const float* b, const float* 
 }
 
 hipError_t nn_head_back(const float* y, const float* target, const float* w, const void* d0, float* dy, float* gd0,
-                        float* gw, float* partial, long long M, int C, hipStream_t s) {
-  head_back_kernel<<<blocks_for(M), 256, 0, s>>>(y, target, w, dy, gd0, M, C, 2.0f / (float)M);
+                        void* gd0_out, const void* mask, float* gw, float* partial, long long M, int C,
+                        hipStream_t s) {
+  head_back_kernel<<<blocks_for(M * C), 256, 0, s>>>(y, target, w, dy, gd0, static_cast<bf16*>(gd0_out),
+                                                 static_cast<const bf16*>(mask), M, C, 2.0f / (float)M);
   const long long ns = (M + 4095) / 4096;
   head_wgrad_kernel<<<(unsigned)ns, 256, 0, s>>>(dy, static_cast<const bf16*>(d0), partial, M, C, 4096);
   reduce_partials_kernel<<<blocks_for(C + 1), 256, 0, s>>>(partial, gw, C + 1, (int)ns);

@@ -196,12 +196,11 @@ class UNet:
         decoder's concatenated input) and bf16 output gradient."""
         L = _lib.lib()
         M = B * h * w
-        need = max(L.sgdct_nn_wgrad_workspace(B, h, w, c.cin, c.cout), L.sgdct_nn_colsum_workspace(M, c.cout))
-        ws = self._workspace(need)
+        ws = self._workspace(L.sgdct_nn_wgrad_workspace(B, h, w, c.cin, c.cout))
         low, x2, c0 = (x_in[0], x_in[1], x_in[0].shape[3]) if isinstance(x_in, tuple) else (x_in, None, 0)
         self._chk(L.sgdct_nn_wgrad_ex(_vp(low), _vp(x2) if x2 is not None else None, c0, _vp(gy), _vp(c.gw),
-                                      _vp(ws), ws.numel(), B, h, w, c.cin, c.cout, self._s()), "wgrad " + c.name)
-        self._chk(L.sgdct_nn_colsum(_vp(gy), _vp(c.gb), _vp(ws), ws.numel(), M, c.cout, self._s()), "colsum")
+                                      _vp(c.gb), _vp(ws), ws.numel(), B, h, w, c.cin, c.cout, self._s()),
+                  "wgrad " + c.name)
 
     def _input_grad(self, c, gy, out, B, h, w, mask=None):
         """Gradient of conv c's input: FP32, or with mask (the input's own
@@ -221,23 +220,24 @@ class UNet:
         target = target.contiguous().float()
         y = self._buf["y"]
         M = B * H * W
+        L_ = self.levels
         dy = self._get("dy", (B, H, W), f32)
-        gd = self._get("gdec0", (B, H, W, self.c0), f32)
+        # Every gradient that ends in a ReLU is written masked, in bf16, by
+        # its last contribution: the head (decoder level 0), the 2x2 sum of
+        # the upsampling's adjoint (the level below in the decoder, and the
+        # bottom encoder level), the pooling's adjoint (encoder levels above
+        # the bottom; their skip part arrives first, from the decoder).
+        gyb = {l: self._get("gy%d_b" % l, tuple(t["dec%d_b" % l].shape), bf) for l in range(L_ - 1)}
+        gyeb = {l: self._get("gye%d_b" % l, tuple(t["enc%d_b" % l].shape), bf) for l in range(L_)}
         ws = self._workspace(L.sgdct_nn_head_workspace(M, self.c0))
-        self._chk(L.sgdct_nn_head_back(_vp(y), _vp(target), _vp(self.head_w), _vp(self.d0), _vp(dy), _vp(gd),
-                                       _vp(self.head_g), _vp(ws), ws.numel(), M, self.c0, self._s()), "head_back")
-        genc = []
-        for l in range(self.levels):
-            g = self._get("genc%d" % l, tuple(t["enc%d_b" % l].shape), f32)
-            self._chk(L.sgdct_nn_fill(_vp(g), g.numel(), 0.0, self._s()), "fill")
-            genc.append(g)
-        gdec = {0: gd}
-        for l in range(0, self.levels - 1):
+        self._chk(L.sgdct_nn_head_back(_vp(y), _vp(target), _vp(self.head_w), _vp(self.d0), _vp(dy), None,
+                                       _vp(gyb[0]), _vp(t["dec0_b"]), _vp(self.head_g), _vp(ws), ws.numel(), M,
+                                       self.c0, self._s()), "head_back")
+        genc = [self._get("genc%d" % l, tuple(t["enc%d_b" % l].shape), f32) for l in range(L_ - 1)]
+        for l in range(0, L_ - 1):
             h, w = H >> l, W >> l
             ca, cb = self.convs["dec%d_a" % l], self.convs["dec%d_b" % l]
-            gy = self._get("gy%d_b" % l, tuple(t["dec%d_b" % l].shape), bf)
-            self._chk(L.sgdct_nn_relu_mask(_vp(gdec[l]), _vp(t["dec%d_b" % l]), _vp(gy), gy.numel(), self._s()),
-                      "relu_mask")
+            gy = gyb[l]
             self._conv_grads(cb, t["dec%d_a" % l], gy, B, h, w)
             gy2 = self._get("gy%d_a" % l, (B, h, w, ca.cout), bf)
             self._input_grad(cb, gy, gy2, B, h, w, mask=t["dec%d_a" % l])
@@ -246,18 +246,16 @@ class UNet:
             self._input_grad(ca, gy2, gin, B, h, w)
             cs = genc[l].shape[3]
             cu = ca.cin - cs
-            glow = genc[self.levels - 1] if l + 1 == self.levels - 1 else self._get(
-                "gdec%d" % (l + 1), (B, h // 2, w // 2, cu), f32)
-            self._chk(L.sgdct_nn_upcat_back(_vp(gin), _vp(glow), _vp(genc[l]), B, h, w, cu, cs, self._s()),
-                      "upcat_back")
-            if l + 1 < self.levels - 1:
-                gdec[l + 1] = glow
-        for l in range(self.levels - 1, -1, -1):
+            if l + 1 < L_ - 1:
+                low_out, low_mask = gyb[l + 1], t["dec%d_b" % (l + 1)]
+            else:
+                low_out, low_mask = gyeb[L_ - 1], t["enc%d_b" % (L_ - 1)]
+            self._chk(L.sgdct_nn_upcat_back(_vp(gin), None, _vp(low_out), _vp(low_mask), _vp(genc[l]), 0, B, h, w,
+                                            cu, cs, self._s()), "upcat_back")
+        for l in range(L_ - 1, -1, -1):
             h, w = H >> l, W >> l
             ca, cb = self.convs["enc%d_a" % l], self.convs["enc%d_b" % l]
-            gy = self._get("gye%d_b" % l, tuple(t["enc%d_b" % l].shape), bf)
-            self._chk(L.sgdct_nn_relu_mask(_vp(genc[l]), _vp(t["enc%d_b" % l]), _vp(gy), gy.numel(), self._s()),
-                      "relu_mask")
+            gy = gyeb[l]
             self._conv_grads(cb, t["enc%d_a" % l], gy, B, h, w)
             gy2 = self._get("gye%d_a" % l, (B, h, w, ca.cout), bf)
             self._input_grad(cb, gy, gy2, B, h, w, mask=t["enc%d_a" % l])
@@ -265,7 +263,8 @@ class UNet:
             if l > 0:
                 gin = self._get("gine%d" % l, (B, h, w, ca.cin), f32)
                 self._input_grad(ca, gy2, gin, B, h, w)
-                self._chk(L.sgdct_nn_pool2_back(_vp(gin), _vp(genc[l - 1]), B, h, w, ca.cin, self._s()),
+                self._chk(L.sgdct_nn_pool2_back(_vp(gin), _vp(genc[l - 1]), _vp(gyeb[l - 1]),
+                                                _vp(t["enc%d_b" % (l - 1)]), B, h, w, ca.cin, self._s()),
                           "pool2_back")
 
     def grads_numpy(self):
