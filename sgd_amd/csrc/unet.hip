@@ -32,6 +32,9 @@ namespace sgdct {
 typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
 
 struct ConvArgs {
   const bf16* in;      // [B][H][W][cin] (with in2: [B][H/2][W/2][c0])
@@ -282,6 +285,167 @@ __global__ __launch_bounds__(256) void conv3x3_lds_kernel(ConvArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Halo-tiled 3x3 convolution, the same implicit GEMM with its operands staged
+// once per block and input-channel chunk: the block's output tile is TH x 16
+// pixels x BN channels, and for each chunk of CC input channels it stages in
+// LDS the (TH + 2) x 18 pixel halo of the tile (80-byte pixel rows for
+// CC = 32) and the chunk's BN x 9 x CC weights (rows of 9 CC + 8 bf16), from
+// which every tap's A and B fragments are read; the next chunk's operands are
+// loaded into registers meanwhile.  Nine taps thus reuse one staged slab
+// instead of re-gathering the tile's pixels per tap.  CC = 32 (cin % 32 == 0):
+// a K step is one tap x 32 channels; CC = 8 (cin == 8): a K step is four
+// taps x 8 channels (lane group g takes tap 4 s + g).  Wave (wm, wn)
+// computes TM tile rows x TN 16-channel tiles.
+// ---------------------------------------------------------------------------
+template <int TH, int BN, int WM, int WN, int CC>
+__global__ __launch_bounds__(256) void conv3x3_halo_kernel(ConvArgs a) {
+  static_assert(WM * WN == 4 && TH % WM == 0, "four waves, whole tile rows per wave");
+  constexpr int TW = 16, HH = TH + 2, HWD = TW + 2;
+  constexpr int PS = CC == 32 ? 40 : CC;          // LDS pixel stride (bf16)
+  constexpr int WR = 9 * CC + 8;                  // LDS weight row (bf16)
+  constexpr int TM = TH / WM, TN = BN / WN / 16;  // MFMA tiles per wave
+  constexpr int GP = CC / 8;                      // 16-byte groups per halo pixel / weight tap
+  constexpr int NG = HH * HWD * GP;               // groups per halo
+  constexpr int NW = BN * 9 * GP;                 // groups per weight slab
+  constexpr int HC = (NG + 255) / 256, WC = (NW + 255) / 256;  // groups per thread
+  constexpr int TPS = 32 / CC;                    // taps per K step
+  constexpr int NKS = (9 + TPS - 1) / TPS;        // K steps per chunk
+  constexpr int ES = BN + 4;                      // epilogue tile row (floats)
+  constexpr int NSTAGE = HH * HWD * PS + BN * WR, NEPI = TH * TW * ES * 2;  // bf16 units
+  __shared__ __attribute__((aligned(16))) bf16 smem[NSTAGE > NEPI ? NSTAGE : NEPI];
+  bf16* const halo = smem;
+  bf16* const wts = smem + HH * HWD * PS;
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = wv % WM, wn = wv / WM;
+  const int r16 = lane & 15, g = lane >> 4;
+  const int tiles_x = (a.W + TW - 1) / TW, tiles_y = (a.H + TH - 1) / TH;
+  int t = blockIdx.x;
+  const int tx = t % tiles_x;
+  t /= tiles_x;
+  const int ty = t % tiles_y, b = t / tiles_y;
+  const int x0 = tx * TW, y0 = ty * TH;
+  const int nb = blockIdx.y * BN;
+  const int K = 9 * a.cin, nchunk = a.cin / CC;
+
+  bf16x8 rh[HC], rw[WC];
+  auto load = [&](int cc) {
+#pragma unroll
+    for (int i = 0; i < HC; ++i) {
+      const int q = tid + 256 * i, p = q / GP, cg = (q % GP) * 8;
+      const int hy = p / HWD, hx = p - hy * HWD;
+      const int iy = y0 + hy - 1, ix = x0 + hx - 1;
+      bf16x8 v = zero8();
+      if (q < NG && iy >= 0 && iy < a.H && ix >= 0 && ix < a.W)
+        v = *reinterpret_cast<const bf16x8*>(conv_in(a.in, a.in2, a.c0, a.H, a.W, a.cin, b, iy, ix, cc * CC + cg));
+      rh[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < WC; ++i) {
+      const int q = tid + 256 * i, n = q / (9 * GP), rem = q - n * (9 * GP);
+      const int tp = rem / GP, cg = (rem - tp * GP) * 8;
+      bf16x8 v = zero8();
+      if (q < NW && nb + n < a.cout)
+        v = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(nb + n) * K + tp * a.cin + cc * CC + cg);
+      rw[i] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < HC; ++i) {
+      const int q = tid + 256 * i, p = q / GP, cg = (q % GP) * 8;
+      if (q < NG) *reinterpret_cast<bf16x8*>(&halo[p * PS + cg]) = rh[i];
+    }
+#pragma unroll
+    for (int i = 0; i < WC; ++i) {
+      const int q = tid + 256 * i, n = q / (9 * GP), rem = q - n * (9 * GP);
+      if (q < NW) *reinterpret_cast<bf16x8*>(&wts[n * WR + rem * 8]) = rw[i];
+    }
+  };
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  // per-lane fragment bases: A at tile row wm TM, pixel r16, channels
+  // (8 g) % CC; B at weight row wn BN / WN + r16
+  const bf16* const abase = &halo[((wm * TM) * HWD + r16) * PS + (8 * g) % CC];
+  const bf16* const bbase = &wts[(wn * (BN / WN) + r16) * WR + (8 * g) % CC];
+  load(0);
+  for (int cc = 0; cc < nchunk; ++cc) {
+    if (cc > 0) __syncthreads();  // the previous chunk's reads come first
+    store();
+    __syncthreads();
+    if (cc + 1 < nchunk) load(cc + 1);
+#pragma unroll 1
+    for (int ks = 0; ks < NKS; ++ks) {
+      // this lane's tap in K step ks and its halo offsets (0..2)
+      const int tg = ks * TPS + (8 * g) / CC;
+      const int dy = tg / 3, dx = tg - 3 * dy;
+      const bool tv = tg < 9;
+      const bf16* const ap = abase + (dy * HWD + dx) * PS;
+      const bf16* const bp = bbase + tg * CC;
+      bf16x8 fa[TM], fb[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) fa[i] = tv ? *reinterpret_cast<const bf16x8*>(ap + i * HWD * PS) : zero8();
+#pragma unroll
+      for (int j = 0; j < TN; ++j) fb[j] = tv ? *reinterpret_cast<const bf16x8*>(bp + 16 * j * WR) : zero8();
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+  }
+
+  // Epilogue through LDS: the block's FP32 tile [pixel][channel] (row stride
+  // BN + 4 floats) replaces the staged operands, then each thread finishes 8
+  // channels of one pixel with 16-byte loads (mask) and stores.
+  constexpr int EG = TH * TW * BN / 8;  // epilogue groups
+  float* const etile = reinterpret_cast<float*>(smem);
+  __syncthreads();
+  // D layout: lane holds rows 4 g + r (pixels of tile row wm TM + i) of column r16
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        etile[((wm * TM + i) * TW + 4 * g + r) * ES + wn * (BN / WN) + 16 * j + r16] = acc[i][j][r];
+  __syncthreads();
+  for (int q = tid; q < EG; q += 256) {
+    const int p = q / (BN / 8), c8 = (q - p * (BN / 8)) * 8;
+    const int oy = y0 + p / TW, ox = x0 + p % TW, co = nb + c8;
+    if (oy >= a.H || ox >= a.W || co >= a.cout) continue;
+    const long long o = (((long long)b * a.H + oy) * a.W + ox) * a.cout + co;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = etile[p * ES + c8 + e] + (a.bias ? a.bias[co + e] : 0.0f);
+      if (a.relu) v[e] = fmaxf(v[e], 0.0f);
+    }
+    if (a.mask) {
+      const i16x8 mk = __builtin_bit_cast(i16x8, *reinterpret_cast<const bf16x8*>(a.mask + o));
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (!(__uint_as_float((unsigned)(unsigned short)mk[e] << 16) > 0.0f)) v[e] = 0.0f;
+    }
+    if (a.out_f32) {
+      float* const op = static_cast<float*>(a.out) + o;
+      *reinterpret_cast<f32x4*>(op) = f32x4{v[0], v[1], v[2], v[3]};
+      *reinterpret_cast<f32x4*>(op + 4) = f32x4{v[4], v[5], v[6], v[7]};
+    } else {
+      bf16x8 h;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) h[e] = (bf16)v[e];
+      *reinterpret_cast<bf16x8*>(static_cast<bf16*>(a.out) + o) = h;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient dW[co][k] = sum_m dY[m][co] X[m + tap(k)][c(k)]: a GEMM
 // whose reduction runs over pixels m.  Block tile CO_T (co) x KT (k) with
 // KT = 256 / CO_T * 16 ... 4 waves, each a 16-row co strip x 64 k (4 MFMA
@@ -309,9 +473,6 @@ struct WgradArgs {
   float* bias_partial;      // [nsplit][cout] column sums of dy, or null
 };
 
-typedef short i16x4 __attribute__((ext_vector_type(4)));
-typedef short i16x8 __attribute__((ext_vector_type(8)));
-typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
 
 // An MFMA fragment of 8 consecutive pixels of one channel from a
 // pixel-major LDS tile: the transposing reads of pixel rows [p, p + 4) and
@@ -739,12 +900,31 @@ __global__ void to_bf16_kernel(const float* __restrict__ src, bf16* __restrict__
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// 3x3 convolutions with cin % 32 == 0 (or cin == 8) and cout % 32 == 0 run
+// the halo-tiled kernel; the others the K-step-staged one.
+constexpr bool kHaloConv = true;
+
 hipError_t nn_conv3x3(const void* in, const void* in2, int c0, const void* w, const float* bias, void* out,
                       const void* mask, int B, int H, int W, int cin, int cout, int relu, int out_f32, hipStream_t s) {
   ConvArgs a{static_cast<const bf16*>(in), static_cast<const bf16*>(w), bias, out, B, H, W, cin, cout, relu, out_f32,
              static_cast<const bf16*>(mask), static_cast<const bf16*>(in2), c0};
   const long long M = (long long)B * H * W;
   if (cin % 8 != 0) return hipErrorInvalidValue;
+  if (kHaloConv && (cin % 32 == 0 || cin == 8) && cout % 32 == 0) {
+    // halo-tiled kernel: 16-row tiles for 32 output channels, 8-row tiles
+    // and 64-channel blocks otherwise
+    const unsigned tiles16 = (unsigned)((W + 15) / 16), img = (unsigned)B;
+    if (cout % 64 == 0) {
+      const dim3 grid(tiles16 * (unsigned)((H + 7) / 8) * img, cout / 64);
+      if (cin == 8) conv3x3_halo_kernel<8, 64, 2, 2, 8><<<grid, 256, 0, s>>>(a);
+      else conv3x3_halo_kernel<8, 64, 2, 2, 32><<<grid, 256, 0, s>>>(a);
+    } else {
+      const dim3 grid(tiles16 * (unsigned)((H + 15) / 16) * img, cout / 32);
+      if (cin == 8) conv3x3_halo_kernel<16, 32, 4, 1, 8><<<grid, 256, 0, s>>>(a);
+      else conv3x3_halo_kernel<16, 32, 4, 1, 32><<<grid, 256, 0, s>>>(a);
+    }
+    return hipGetLastError();
+  }
   if (cout % 128 == 0) {
     conv3x3_lds_kernel<128, 128, 2, 2><<<dim3(blocks_for(M, 128), cout / 128), 256, 0, s>>>(a);
   } else if (cout % 64 == 0) {
