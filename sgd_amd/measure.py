@@ -8,6 +8,8 @@
                                       # 360 views, 512 x 384 detector
     python -m sgd_amd.measure unet    # U-Net (4 levels, 32 channels) inference and
                                       # training step, batch 8 at 512 x 512
+    python -m sgd_amd.measure conv    # the U-Net's 3x3 convolutions alone (32 -> 32 and
+                                      # 64 -> 64 channels, batch 8 at 512 x 512)
 
 Times are HIP-event averages on the launch stream with inputs resident in
 device memory.
@@ -156,6 +158,30 @@ def cone(reps=3):
     return recs
 
 
+def conv(reps=10, batch_size=8, size=512):
+    """sgdct_nn_conv3x3_ex, bias + ReLU, bf16 output."""
+    import ctypes
+    from . import _lib
+    L = _lib.lib()
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    recs = []
+    for ci, co in ((32, 32), (64, 64)):
+        x = torch.rand(batch_size, size, size, ci, device="cuda").to(torch.bfloat16)
+        w = (torch.rand(co, 9 * ci, device="cuda") * 0.1).to(torch.bfloat16)
+        b = torch.zeros(co, device="cuda")
+        y = torch.empty(batch_size, size, size, co, device="cuda", dtype=torch.bfloat16)
+
+        def run():
+            _lib.check(L.sgdct_nn_conv3x3_ex(vp(x), None, 0, vp(w), vp(b), vp(y), None, batch_size, size, size, ci,
+                                             co, 1, 0, None), "conv3x3")
+        ms = _time(run, reps)
+        flop = 2.0 * 9 * ci * co * batch_size * size * size
+        recs.append(dict(workload="3x3 convolution %d -> %d channels, batch %d at %dx%d, bf16" % (ci, co, batch_size,
+                                                                                                  size, size),
+                         us=ms * 1e3, tflops=flop / ms / 1e9))
+    return recs
+
+
 def unet(reps=5, batch_size=8, size=512):
     from .unet import UNet, layer_specs
     net = UNet(4, 32, seed=0)
@@ -177,13 +203,15 @@ def unet(reps=5, batch_size=8, size=512):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("what", choices=["fan", "asd", "batch", "cone", "unet"])
+    ap.add_argument("what", choices=["fan", "asd", "batch", "cone", "unet", "conv"])
     ap.add_argument("--tune-cache", default=None)
     a = ap.parse_args(argv)
     if a.what == "batch":
         recs = [batch(tune_cache=a.tune_cache)]
     elif a.what == "cone":
         recs = cone()
+    elif a.what == "conv":
+        recs = conv()
     elif a.what == "fan":
         recs = fan()
     elif a.what == "asd":
