@@ -188,12 +188,14 @@ def _norms(t):
 
 
 class AsdPocsBatchResult:
-    def __init__(self, x, iters, residuals, fell_back, seconds):
+    def __init__(self, x, iters, residuals, fell_back, seconds, history=None):
         self.x, self.iters, self.residuals, self.fell_back, self.seconds = x, iters, residuals, fell_back, seconds
+        self.history = history
 
 
 def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, beta_red=0.995, ng=20, alpha=0.2,
-                   r_max=0.95, alpha_red=0.95, eps_data=0.0, tv_eps=1e-8, weights=None, check_every=10):
+                   r_max=0.95, alpha_red=0.95, eps_data=0.0, tv_eps=1e-8, weights=None, check_every=10,
+                   history=False):
     """ASD-POCS (Sidky & Pan 2008) on a batch of sinograms b (B, np, nu) with
     per-image TV step control and stopping.  An image stops at the first
     iteration whose entering residual ||A x - b|| / ||b|| is <= tol (a number,
@@ -202,6 +204,9 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
     `fallback` (B, ny, nx), an image whose residual rises between two
     iterations of its warm-started run restarts once from its fallback image.
     The data step's relaxation beta follows one schedule for the batch.
+    With history=True (and no tol) the result's `history` holds every
+    iteration's residual, (niter, B): entry k is the residual of the iterate
+    after k iterations (entry 0 that of the starting image).
 
     Stopped images leave the working batch at the next check (every
     `check_every` iterations), so the work per iteration follows the number
@@ -230,10 +235,15 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
     dtvg = torch.full((nb,), float("nan"), dtype=torch.float64, device=dev)
     tolw = None if tol is None else torch.as_tensor(tol, dtype=torch.float64, device=dev).expand(nb).clone()
     g = torch.empty_like(x)
+    if history and tol is not None:
+        raise ValueError("asd_pocs_batch: history needs tol=None (every image runs all iterations)")
+    hist = [] if history else None
     for k in range(niter):
         x_prev = x.clone()
         ax = sirt_step(P, w, x, bw, beta, positivity=True)
         res = _norms(ax - bw) / bn
+        if hist is not None:
+            hist.append(res)
         last_res[idx] = torch.where(active, res, last_res[idx])
         if tolw is not None:
             done = active & (res <= tolw)
@@ -273,7 +283,8 @@ def asd_pocs_batch(P, b, niter=200, tol=None, x0=None, fallback=None, beta=1.0, 
     fell[idx] = fellw
     torch.cuda.synchronize()
     return AsdPocsBatchResult(x_out, iters.cpu().numpy(), last_res.cpu().numpy(), fell.cpu().numpy(),
-                              time.perf_counter() - t0)
+                              time.perf_counter() - t0,
+                              torch.stack(hist).cpu().numpy() if hist else None)
 
 
 def sirt_batch(P, b, niter, weights=None, x0=None, lam=1.0):

@@ -118,8 +118,20 @@ def make_pairs(P, seeds, batch=64, sirt_iters=5, asd_iters=200, log=print):
     return torch.cat(ins), torch.cat(outs), torch.cat(sinos)
 
 
-def train(net, inputs, targets, epochs=8, batch=8, lr=1e-3, seed=0, log=print):
+def train(net, inputs, targets, epochs=8, batch=8, lr=1e-3, seed=0, log=print, P=None, sinos=None,
+          data_weight=0.0, weights=None):
+    """Adam on mean((y - target)^2) with a cosine learning rate.  With
+    data_weight = lam > 0 (and the projector P and the pairs' sinograms) the
+    output gradient becomes that of
+        mean((y - t)^2) + lam * (the SIRT-metric data misfit of y),
+    i.e. (y - t) + lam * g with g = V A^T W (A y - b), the negative of one
+    unconstrained SIRT step from y: the network is steered towards outputs
+    whose data residual is small, not only towards the target pixels."""
+    from .solvers import sirt_step
     n = inputs.shape[0]
+    if data_weight > 0 and (P is None or sinos is None):
+        raise ValueError("train: data_weight > 0 needs the projector P and the sinograms")
+    w = (weights or SirtWeights(P)) if data_weight > 0 else None
     rng = np.random.default_rng(seed)
     losses = []
     steps = epochs * (n // batch)
@@ -132,8 +144,16 @@ def train(net, inputs, targets, epochs=8, batch=8, lr=1e-3, seed=0, log=print):
         for k in range(0, n - batch + 1, batch):
             idx = torch.as_tensor(order[k:k + batch], device=inputs.device)
             lr_s = lr * 0.5 * (1 + np.cos(np.pi * s / steps))
-            y = net.train_step(inputs[idx], targets[idx], lr=lr_s)
-            loss = ((y - targets[idx]).double() ** 2).mean()
+            t = targets[idx]
+            if data_weight > 0:
+                y = net.forward(inputs[idx])
+                z = y.clone()
+                sirt_step(P, w, z, sinos[idx].contiguous(), 1.0, positivity=False)  # z = y - g
+                net.backward(t + data_weight * (z - y))
+                net.step(lr_s)
+            else:
+                y = net.train_step(inputs[idx], t, lr=lr_s)
+            loss = ((y - t).double() ** 2).mean()
             run += loss
             nstep += 1
             if s % 50 == 0:
@@ -143,12 +163,15 @@ def train(net, inputs, targets, epochs=8, batch=8, lr=1e-3, seed=0, log=print):
     return losses
 
 
-def evaluate(P, net, b, tols=("2s", "1.5s", "1e-2"), max_iters=3000, batch=64, sirt_iters=5, log=print):
+def evaluate(P, net, b, tols=("2s", "1.5s", "1e-2"), max_iters=3000, batch=64, sirt_iters=5, log=print,
+             plain_cache=None):
     """Plain ASD-POCS (from zero) vs warm-started (from UNet(SIRT(5)), with
     the SIRT(5) image as fallback) to each relative-residual tolerance in
     `tols` (see parse_tol: a number, or a multiple of each image's own noise
     level, the discrepancy principle): per-image iterations and wall times per
-    method and tolerance.  The warm time includes SIRT(5) and the network."""
+    method and tolerance.  The warm time includes SIRT(5) and the network.
+    A dict passed as plain_cache keeps the plain runs for the next call on
+    the same sinograms (the plain method does not depend on the network)."""
     w = SirtWeights(P)
     specs = [parse_tol(t) for t in tols]
     res = {name: dict(plain_iters=[], warm_iters=[], plain_s=0.0, warm_s=0.0, fell_back=0, tol=[])
@@ -164,7 +187,14 @@ def evaluate(P, net, b, tols=("2s", "1.5s", "1e-2"), max_iters=3000, batch=64, s
         t_seed = time.perf_counter() - t0
         for name, val, rel in specs:
             tol = val * sigma if rel else val
-            r = asd_pocs_batch(P, bb, niter=max_iters, tol=tol, weights=w, **ASD)
+            key = (k, name, max_iters)
+            if plain_cache is not None and key in plain_cache:
+                r = plain_cache[key]
+            else:
+                r = asd_pocs_batch(P, bb, niter=max_iters, tol=tol, weights=w, **ASD)
+                r.x = None
+                if plain_cache is not None:
+                    plain_cache[key] = r
             r2 = asd_pocs_batch(P, bb, niter=max_iters, tol=tol, x0=x0, fallback=xs, weights=w, **ASD)
             e = res[name]
             e["plain_s"] += r.seconds
@@ -194,6 +224,87 @@ def summarize(ev, max_iters):
             plain_s=e["plain_s"], warm_s=e["warm_s"],
             speedup_iterations=float(pi.mean() / max(wi.mean(), 1e-9)),
             speedup_time=float(e["plain_s"] / max(e["warm_s"], 1e-9)), fell_back=e["fell_back"])
+    return out
+
+
+def curve_stats(hp, hw, sig, tol=1e-2, marks=(0, 1, 2, 5, 10, 20, 50, 100, 150, 200, 300, 500, 750, 1000, 1499)):
+    """Statistics of residual curves hp, hw (iterations, images) of the plain
+    and the warm-started run and the images' noise levels sig (images,):
+    see residual_curves."""
+    iters, n = hp.shape
+    pmin = np.minimum.accumulate(hp, axis=0)  # non-increasing envelope of each plain curve
+    marks = [m for m in marks if m < iters]
+    out = dict(images=n, iters=iters, tol=tol, noise_level_min=float(sig.min()), noise_level_max=float(sig.max()),
+               marks=marks, plain_median=[], warm_median=[], plain_median_over_noise=[],
+               warm_median_over_noise=[], offset_median=[], offset_censored=[])
+    for m in marks:
+        out["plain_median"].append(float(np.median(hp[m])))
+        out["warm_median"].append(float(np.median(hw[m])))
+        out["plain_median_over_noise"].append(float(np.median(hp[m] / sig)))
+        out["warm_median_over_noise"].append(float(np.median(hw[m] / sig)))
+        d = []
+        for i in range(n):
+            j = np.nonzero(pmin[:, i] <= hw[m, i])[0]
+            if j.size:
+                d.append(int(j[0]) - m)
+        # median over the images whose plain run gets as low within iters
+        out["offset_median"].append(float(np.median(d)) if d else None)
+        out["offset_censored"].append(n - len(d))
+
+    def first(h):
+        r = np.full(n, iters, dtype=np.int64)
+        for i in range(n):
+            j = np.nonzero(h[:, i] <= tol)[0]
+            if j.size:
+                r[i] = j[0]
+        return r
+    fp, fw = first(hp), first(hw)
+    both = (fp < iters) & (fw < iters)
+    out.update(tol_plain_reached=int((fp < iters).sum()), tol_warm_reached=int((fw < iters).sum()),
+               tol_plain_mean=float(fp.mean()), tol_warm_mean=float(fw.mean()),
+               tol_plain_median=float(np.median(fp)), tol_warm_median=float(np.median(fw)),
+               tol_both_plain_mean=float(fp[both].mean()) if both.any() else None,
+               tol_both_warm_mean=float(fw[both].mean()) if both.any() else None,
+               tol_saved_median=float(np.median((fp - fw)[both])) if both.any() else None,
+               final_plain_over_noise=[float(v) for v in np.quantile(hp[-1] / sig, (0.0, 0.5, 1.0))])
+    return out
+
+
+def residual_curves(P, net, b, iters=1500, batch=64, sirt_iters=5, tol=1e-2,
+                    marks=(0, 1, 2, 5, 10, 20, 50, 100, 150, 200, 300, 500, 750, 1000, 1499), log=print,
+                    plain_cache=None):
+    """Residual curves of plain and warm-started ASD-POCS run for `iters`
+    iterations without stopping (same controls and seeds as evaluate), and
+    what they say about a fixed tolerance:
+
+      median residual and residual / own noise level at the `marks`;
+      offset d(k) = E(k) - k, where E(k) is the first plain iteration whose
+        residual is at or below the warm residual after k iterations (the
+        plain iterations the warm start has saved by iteration k); censored
+        when the plain run never gets that low within `iters` (the median is
+        over the other images, None when all are censored);
+      iterations to `tol` read off the curves (first k with residual <= tol),
+        for both methods.
+    plain_cache: as in evaluate."""
+    w = SirtWeights(P)
+    hp, hw, sig = [], [], []
+    for k in range(0, b.shape[0], batch):
+        bb = b[k:k + batch]
+        sig.append(noise_level(bb).cpu().numpy())
+        xs = sirt_batch(P, bb, sirt_iters, weights=w)
+        x0 = net.forward(xs).clone()
+        key = ("curve", k, iters)
+        if plain_cache is not None and key in plain_cache:
+            hp.append(plain_cache[key])
+        else:
+            hp.append(asd_pocs_batch(P, bb, niter=iters, weights=w, history=True, **ASD).history)
+            if plain_cache is not None:
+                plain_cache[key] = hp[-1]
+        hw.append(asd_pocs_batch(P, bb, niter=iters, x0=x0, weights=w, history=True, **ASD).history)
+        log("curves %d/%d" % (k + bb.shape[0], b.shape[0]))
+    hp, hw, sig = np.concatenate(hp, axis=1), np.concatenate(hw, axis=1), np.concatenate(sig)
+    out = curve_stats(hp, hw, sig, tol=tol, marks=marks)
+    log(json.dumps(out))
     return out
 
 
@@ -235,8 +346,9 @@ def load_weights(path, levels=4, c0=32):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--phase", choices=["all", "train", "eval", "floor"], default="all",
+    ap.add_argument("--phase", choices=["all", "train", "eval", "floor", "curves"], default="all",
                     help="train: generate pairs, train, save --weights; eval: load --weights, evaluate; "
+                         "curves: load --weights, residual curves (--curves images); "
                          "floor: residual floor of SIRT / ASD-POCS on 8 phantoms")
     ap.add_argument("--weights", default=None, help="weights file (.npz) written by train / read by eval")
     ap.add_argument("--pairs", type=int, default=4096)
@@ -250,10 +362,18 @@ def main(argv=None):
     ap.add_argument("--max-iters", type=int, default=3000)
     ap.add_argument("--asd-iters", type=int, default=200)
     ap.add_argument("--gen-batch", type=int, default=64)
+    ap.add_argument("--curves", type=int, default=0,
+                    help="residual curves on this many test images (after eval with --phase all)")
+    ap.add_argument("--curve-iters", type=int, default=1500)
+    ap.add_argument("--data-weight", default="0",
+                    help="comma-separated data-consistency weights of the training loss (see train); one "
+                         "network is trained per weight on the same pairs and each is evaluated")
     ap.add_argument("--out", default="warmstart_report.json")
     a = ap.parse_args(argv)
-    if a.phase == "eval" and not a.weights:
-        ap.error("--phase eval needs --weights")
+    if a.phase in ("eval", "curves") and not a.weights:
+        ap.error("--phase %s needs --weights" % a.phase)
+    if a.phase == "curves" and a.curves <= 0:
+        a.curves = 64
 
     def log(msg):
         print(msg, flush=True)
@@ -267,33 +387,53 @@ def main(argv=None):
     tols = a.tol.split(",")
     report = dict(workload=dict(GEOMETRY, mu=MU, n0=N0, asd=ASD, pairs=a.pairs, test=a.test, tols=tols,
                                 max_iters=a.max_iters, asd_iters_target=a.asd_iters, epochs=a.epochs,
-                                batch=a.batch, lr=a.lr),
+                                batch=a.batch, lr=a.lr, data_weight=a.data_weight),
                   phase=a.phase, seconds={})
+    dws = [float(v) for v in a.data_weight.split(",")]
+    nets = []
+    report["variants"] = {}
     if a.phase in ("all", "train"):
         t0 = time.perf_counter()
-        xin, xt, _ = make_pairs(P, list(range(a.pairs)), batch=a.gen_batch, asd_iters=a.asd_iters, log=log)
+        xin, xt, sinos = make_pairs(P, list(range(a.pairs)), batch=a.gen_batch, asd_iters=a.asd_iters, log=log)
         report["seconds"]["generate"] = time.perf_counter() - t0
-        net = UNet(4, 32, seed=0)
-        t0 = time.perf_counter()
-        losses = train(net, xin, xt, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
-        report["seconds"]["train"] = time.perf_counter() - t0
-        report["train_loss"] = losses[-5:]
-        if a.weights:
-            save_weights(net, a.weights)
-        del xin, xt
+        for dw in dws:
+            label = "data_weight=%g" % dw
+            net = UNet(4, 32, seed=0)
+            t0 = time.perf_counter()
+            log("training, " + label)
+            losses = train(net, xin, xt, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, P=P, sinos=sinos,
+                           data_weight=dw)
+            report["variants"][label] = dict(train_seconds=time.perf_counter() - t0, train_loss=losses[-5:])
+            if a.weights:
+                save_weights(net, a.weights if len(dws) == 1 else a.weights.replace(".npz", "_dw%g.npz" % dw))
+            nets.append((label, net))
+        del xin, xt, sinos
     else:
-        net = load_weights(a.weights)
-    if a.phase in ("all", "eval"):
-        seeds = list(range(10 ** 6, 10 ** 6 + a.test))
+        nets.append(("weights=" + a.weights, load_weights(a.weights)))
+        report["variants"][nets[0][0]] = {}
+    if a.phase in ("all", "eval", "curves"):
+        ntest = max(a.test, a.curves) if a.phase != "curves" else a.curves
+        seeds = list(range(10 ** 6, 10 ** 6 + ntest))
         x = torch.cat([phantoms(seeds[k:k + 64], P.nx, P.ny, P.geom["dx"], P.geom["dy"], P.device)
                        for k in range(0, len(seeds), 64)])
         b = torch.cat([noisy_sinograms(P, x[k:k + 64], seed=seeds[k] + 7919) for k in range(0, len(seeds), 64)])
         del x
-        ev = evaluate(P, net, b, tols=tols, max_iters=a.max_iters, log=log)
-        report["tolerances"] = summarize(ev, a.max_iters)
+        plain_cache = {}
+        for label, net in nets:
+            v = report["variants"][label]
+            if a.phase != "curves" and a.test > 0:
+                t0 = time.perf_counter()
+                ev = evaluate(P, net, b[:a.test], tols=tols, max_iters=a.max_iters, log=log, plain_cache=plain_cache)
+                v["evaluate_seconds"] = time.perf_counter() - t0
+                v["tolerances"] = summarize(ev, a.max_iters)
+            if a.curves > 0:
+                t0 = time.perf_counter()
+                v["curves"] = residual_curves(P, net, b[:a.curves], iters=a.curve_iters, log=log,
+                                              plain_cache=plain_cache)
+                v["curves_seconds"] = time.perf_counter() - t0
     with open(a.out, "w") as f:
         json.dump(report, f, indent=1)
-    log(json.dumps({k: report[k] for k in ("phase", "seconds", "tolerances") if k in report}))
+    log(json.dumps({k: report[k] for k in ("phase", "seconds", "variants") if k in report}))
     return report
 
 
