@@ -59,15 +59,18 @@ __device__ __forceinline__ float sf_weight(float dx, float wx, float wy) {
 }
 
 // A: bin-driven gather (same strip walk as fwd_bin_kernel, corner rule h = 1/2).
-// One wave: bins [64 chunk, 64 chunk + 64) of view v, wave = v * nchunk + chunk.
+// One wave: bins [64 chunk, 64 chunk + 64) of view v, wave = chunk * np + v
+// (chunk-major: the four waves of a workgroup walk the same bins of adjacent
+// views, nearly the same pixels; SF A 1.505 -> 1.468 ms at 512 x 512, 720
+// views, 768 bins against view-major order).
 // nseg > 1: lines [seg L, seg L + L) only, FP64 partial sum to part[v nu + u].
 __device__ __forceinline__ void sf_fwd_wave(const Proj2DArgs& g, const float* __restrict__ img,
                                             float* __restrict__ sino, int wave, int seg = 0, int nseg = 1,
                                             double* __restrict__ part = nullptr) {
   const int lane = threadIdx.x & 63;
-  const int v = wave / g.nchunk;
-  if (v >= g.np) return;
-  const int chunk = wave - v * g.nchunk;
+  const int chunk = wave / g.np;
+  if (chunk >= g.nchunk) return;
+  const int v = wave - chunk * g.np;
   const int u = chunk * 64 + lane;
   const ViewGeo vg = g.views[v];
   const int umid = min(chunk * 64 + 32, g.nu - 1);

@@ -10,6 +10,8 @@
                                       # training step, batch 8 at 512 x 512
     python -m sgd_amd.measure conv    # the U-Net's 3x3 convolutions alone (32 -> 32 and
                                       # 64 -> 64 channels, batch 8 at 512 x 512)
+    python -m sgd_amd.measure hexangle  # hexagonal A by view angle (1024 views within
+                                        # 0.06 degrees of each angle, batch 8)
 
 Times are HIP-event averages on the launch stream with inputs resident in
 device memory.
@@ -115,6 +117,28 @@ def fan(reps=10):
     return recs
 
 
+def hexangle(reps=5, angles=(0, 5, 10, 15, 20, 25, 30, 45, 60, 90, 120, 150, 165)):
+    """Hexagonal A (H0, Hl) by view angle: 1024 views within +-0.03 degrees
+    of each angle, 296 x 256 lattice (dt 0.5 mm), 1024 bins of 0.6 mm, lso
+    500 / lsd 1000 mm, batch 8 of uniform random images; ms per image."""
+    from .torch_ops import Projector
+    recs = []
+    x = torch.rand((8, 296, 256), device=torch.device("cuda", torch.cuda.current_device()))
+    for basis in ("Hl", "H0"):
+        ms = {}
+        for adeg in angles:
+            ang = np.radians(adeg + np.linspace(-0.03, 0.03, 1024))
+            P = Projector.hexagonal(basis, 296, 256, ang, 1024, 0.5, 0.6, 1000.0, 500.0)
+            yo = torch.empty((8,) + tuple(P.sino_shape), device=x.device)
+            ms[str(adeg)] = _time(lambda: P.forward(x, out=yo), reps) / 8
+            del P
+        v = np.array(list(ms.values()))
+        recs.append(dict(workload="%s A by view angle, 296x256 hexagonal lattice (dt 0.5 mm), 1024 views within "
+                                  "0.06 deg of each angle, 1024 bins (0.6 mm), batch 8" % basis,
+                         basis=basis, A_ms_per_image=ms, max_over_median=float(v.max() / np.median(v))))
+    return recs
+
+
 def asd(niter=50):
     """Plain ASD-POCS on B1 data: 512^2 phantom (0.6 mm), 720 views,
     768 bins (0.8 mm), Poisson noise at 1e5 photons on p = 0.1 A x, 50
@@ -203,7 +227,7 @@ def unet(reps=5, batch_size=8, size=512):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("what", choices=["fan", "asd", "batch", "cone", "unet", "conv"])
+    ap.add_argument("what", choices=["fan", "asd", "batch", "cone", "unet", "conv", "hexangle"])
     ap.add_argument("--tune-cache", default=None)
     a = ap.parse_args(argv)
     if a.what == "batch":
@@ -214,6 +238,8 @@ def main(argv=None):
         recs = conv()
     elif a.what == "fan":
         recs = fan()
+    elif a.what == "hexangle":
+        recs = hexangle()
     elif a.what == "asd":
         recs = [asd()]
     else:
