@@ -80,6 +80,23 @@ struct BatchPlan {
   long long nfwd, nbwd;
 };
 
+// Workgroup -> work item (x, y, z) of the grid, walked x fastest, such that
+// the workgroups one XCD receives take a contiguous range of items.  The
+// dispatcher deals the workgroups of a launch to the 8 XCDs in turn
+// (workgroup n in dispatch order to XCD n mod 8); in the gather kernels z
+// selects a view group, whose bin records and sinogram rows every x-y tile of
+// that z reads, so each XCD's L2 then holds the records of its own groups
+// instead of all of them.  Any grid: XCD k takes items [k q + min(k, r), ...)
+// with q = n / 8, r = n % 8.  Used where it measured faster (SF and H0 A^T).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ uint3 xcd_block() {
+  const unsigned tx = gridDim.x, ty = gridDim.y, n = tx * ty * gridDim.z;
+  const unsigned lin = blockIdx.x + tx * (blockIdx.y + ty * blockIdx.z);
+  const unsigned k = lin % kXcds, q = n / kXcds, r = n % kXcds;
+  const unsigned item = k * q + (k < r ? k : r) + lin / kXcds;
+  return make_uint3(item % tx, (item / tx) % ty, item / (tx * ty));
+}
+
 constexpr int kMaxWindow = 128;   // bins of one view cached in LDS per pixel tile
 constexpr int kViewChunk = 4;     // views per LDS refill in the gather kernels
 constexpr int kTile = 16;         // pixel tile edge of the gather kernels
