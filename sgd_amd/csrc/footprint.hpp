@@ -100,25 +100,4 @@ __device__ __forceinline__ float b1_mean(const B1Bin& r, float ta, float tb) {
   return mean_over(s_b1(ta, r), s_b1(tb, r), ta, tb);
 }
 
-// The same with 1 / (tb - ta) supplied by the caller.
-__device__ __forceinline__ float b1_mean_inv(const B1Bin& r, float ta, float tb, float inv_w) {
-  return (s_b1(tb, r) - s_b1(ta, r)) * inv_w;
-}
-
-// Along one line of A's walk tb - ta is affine in the pixel index and changes
-// by a relative delta per pixel of the pixel pitch over the distance from the
-// source (~1e-3 at 0.6 mm pixels and 500 mm, ~5e-3 at 2.4 mm).  From the
-// previous pixel's reciprocal r, with e = 1 - w' r,
-//   r' = r (1 + e + e^2)
-// has the relative error (delta + err(r))^3 (it does not accumulate along the
-// line): ~1e-7 with FP32 rounding at 5e-3.  Three full-rate FMAs replace the
-// quarter-rate v_rcp_f32 (the walk takes one v_rcp_f32 per line): B1 A 0.952
-// -> 0.946 ms, B0 0.626 -> 0.619 ms at 512 x 512, 720 views, 768 bins.  The
-// two-FMA Newton step r (1 + e) errs by delta^2 (2e-5 at 2.4 mm pixels) for
-// 0.3 % more.
-__device__ __forceinline__ float next_inv(float inv_w, float w_next) {
-  const float e = __builtin_fmaf(-w_next, inv_w, 1.0f);
-  return __builtin_fmaf(inv_w, __builtin_fmaf(e, e, e), inv_w);
-}
-
 }  // namespace sgdct

@@ -21,10 +21,6 @@ template <> struct Traits<BASIS_B1> {
   using Bin = B1Bin;
   // times scale(r) it is the footprint
   __device__ static __forceinline__ float conv(const Bin& r, float ta, float tb) { return b1_mean(r, ta, tb); }
-  // the same, 1 / (tb - ta) given
-  __device__ static __forceinline__ float conv_inv(const Bin& r, float ta, float tb, float inv_w) {
-    return b1_mean_inv(r, ta, tb, inv_w);
-  }
   __host__ __device__ static __forceinline__ float scale(const Bin& r) { return r.scale; }
   // s of the scaled offset t' = s t: scale = s c' and h = 1 / (2 c') (the
   // tri_a * box record: scale = s, h = 1/2)
@@ -37,9 +33,6 @@ template <> struct Traits<BASIS_B0> {
     float sa = s_b0(ta, r.hpq, r.hmq, r.inv_p, r.inv_2pq);
     float sb = s_b0(tb, r.hpq, r.hmq, r.inv_p, r.inv_2pq);
     return mean_over(sa, sb, ta, tb);
-  }
-  __device__ static __forceinline__ float conv_inv(const Bin& r, float ta, float tb, float inv_w) {
-    return (s_b0(tb, r.hpq, r.hmq, r.inv_p, r.inv_2pq) - s_b0(ta, r.hpq, r.hmq, r.inv_p, r.inv_2pq)) * inv_w;
   }
   __host__ __device__ static __forceinline__ float scale(const Bin&) { return 1.0f; }
   __host__ __device__ static __forceinline__ float tscale(const Bin&) { return 1.0f; }
@@ -139,16 +132,12 @@ __device__ __forceinline__ void fwd_bin_wave(const Proj2DArgs& g, const float* _
     unsigned off = 4u * (unsigned)(m * smaj + n0 * smin);
     const unsigned end = off + (unsigned)((int)fhi - n0 + 1) * bstep;
     float accl = 0.0f;
-    // 1 / (tb - ta): v_rcp_f32 at the line start, then a third-order update
-    // per pixel from the previous value (next_inv)
-    float inv_w = __builtin_amdgcn_rcpf(tb - ta);
     do {
       const float x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(img_rsrc, (int)off, 0, 0));
-      accl = __builtin_fmaf(T::conv_inv(rec, ta, tb, inv_w), x, accl);
+      accl = __builtin_fmaf(T::conv(rec, ta, tb), x, accl);
       off += bstep;
       ta += dta;
       tb += dtb;
-      inv_w = next_inv(inv_w, tb - ta);
     } while (off != end);
     acc += (double)accl;
   };
