@@ -158,6 +158,10 @@ void sgdct_destroy(sgdct_projector *h);
  * a hipStream_t (NULL = default stream).  Asynchronous on `stream`.  Cone-beam
  * handles, and fan-beam handles on launches they split (below), keep per-handle
  * workspace: calls on one handle must not overlap (issue them on one stream).
+ * A cone-beam A that takes more than one pass of views (a pass holds up to
+ * 2 GiB of row profiles) also runs every other pass on a stream of the
+ * handle's own, joined back to `stream` by events before the call returns;
+ * its second 2 GiB buffer is allocated with the rest of the workspace.
  * Returns SGDCT_OK or a negative code. */
 int sgdct_forward_dev(sgdct_projector *h, const float *img, float *sino, int batch, void *stream);
 int sgdct_backward_dev(sgdct_projector *h, const float *sino, float *img, int batch, void *stream);
