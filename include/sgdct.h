@@ -251,8 +251,10 @@ typedef struct sgdct_batch sgdct_batch;
 sgdct_batch *sgdct_batch_create(sgdct_projector *const *slices, int nslice);
 void sgdct_batch_destroy(sgdct_batch *b);
 int sgdct_batch_layout(const sgdct_batch *b, long long *img_off, long long *sino_off);
-/* Launch shape: persistent blocks per CU for A and A^T (1..32) and slice
- * order (0: decreasing per-item cost, 1: as given); set by the autotuner. */
+/* Launch shape: persistent blocks per CU for A and A^T (1..32, or 0 for one
+ * launch per slice through the slices' own projector kernels and workspaces)
+ * and slice order of the persistent queue (0: decreasing per-item cost, 1: as
+ * given); set by the autotuner.  Defaults: 8, 8, 0. */
 int sgdct_batch_config(sgdct_batch *b, int fwd_blocks_per_cu, int bwd_blocks_per_cu, int order);
 int sgdct_batch_forward(sgdct_batch *b, const float *img, float *sino, void *stream);
 int sgdct_batch_backward(sgdct_batch *b, const float *sino, float *img, void *stream);

@@ -9,7 +9,9 @@ kernel drains (see sgd_amd/csrc/batch.hip).  Images (and sinograms) of all
 slices live back to back in one device buffer at `img_off` (`sino_off`).
 
 `tune()` times the candidate launch shapes -- persistent blocks per CU for A
-and A^T, and the slice order (decreasing per-item cost, or as given) -- with
+and A^T, or one launch per slice through each projector's own split kernels
+(blocks per CU 0), and the slice order of the persistent queue (decreasing
+per-item cost, or as given) -- with
 HIP events on the current stream and caches the winner per device and
 workload signature in a JSON file ($SGDCT_TUNE_CACHE, default
 ~/.cache/sgd_amd/autotune.json).
@@ -25,8 +27,9 @@ import torch
 from . import _lib
 from .torch_ops import Projector
 
-FWD_BLOCKS = (2, 4, 8, 16)
-BWD_BLOCKS = (1, 2, 4, 8)
+# candidate persistent blocks per CU; 0 = one launch per slice
+FWD_BLOCKS = (0, 2, 4, 8, 16)
+BWD_BLOCKS = (0, 1, 2, 4, 8)
 ORDERS = (0, 1)
 
 
@@ -84,7 +87,7 @@ class SliceBatch:
         _lib.check(L.sgdct_batch_layout(h, io, so), "batch layout")
         self.img_off, self.sino_off = list(io), list(so)
         self.device = self.projectors[0].device
-        self.config = (8, 4, 0)
+        self.config = (8, 8, 0)
 
     def close(self):
         if getattr(self, "handle", None):

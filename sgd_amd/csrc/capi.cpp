@@ -213,6 +213,7 @@ struct sgdct_batch {
   int n = 0, device = 0, ncu = 256;
   std::vector<long long> img_off, sino_off;  // [n + 1]
   std::vector<SliceDesc> desc;
+  std::vector<sgdct_projector*> proj;        // the slices' handles (per-slice mode)
   std::vector<double> fwd_cost, bwd_cost;    // per-item cost estimates
   std::vector<long long> fwd_items, bwd_items;
   SliceDesc* d_slices = nullptr;
@@ -220,7 +221,11 @@ struct sgdct_batch {
   long long* d_prefix = nullptr;   // [2][n + 1]
   unsigned int* d_counters = nullptr;
   BatchPlan plan{};
-  int fwd_bpc = 8, bwd_bpc = 4, order = 0;
+  // Persistent blocks per CU of A and A^T; 0 = one launch per slice (each
+  // projector's own split kernels).  Defaults: the autotuner's persistent
+  // choice on 256 uneven slices (A^T at 8 blocks per CU: 165.5 ms against
+  // 200.8 ms at 4).
+  int fwd_bpc = 8, bwd_bpc = 8, order = 0;
   struct Graph {
     hipGraphExec_t exec = nullptr;
     const void* in = nullptr;
@@ -821,6 +826,16 @@ int batch_upload_plan(sgdct_batch* b) {
 }
 
 int batch_run(sgdct_batch* b, int dir, const float* in, float* out, hipStream_t s) {
+  if ((dir == 0 ? b->fwd_bpc : b->bwd_bpc) == 0) {
+    // one launch per slice, in slice order, on the slices' own handles
+    for (int i = 0; i < b->n; ++i) {
+      sgdct_projector* h = b->proj[i];
+      const int rc = dir == 0 ? run_forward(h, in + b->img_off[i], out + b->sino_off[i], 1, s)
+                              : run_backward(h, in + b->sino_off[i], out + b->img_off[i], 1, s);
+      if (rc) return rc;
+    }
+    return SGDCT_OK;
+  }
   hipError_t e = dir == 0 ? launch_batch_forward(b->plan, b->ncu * b->fwd_bpc, in, out, s)
                           : launch_batch_backward(b->plan, b->ncu * b->bwd_bpc, in, out, s);
   return e == hipSuccess ? SGDCT_OK : hip_fail(e, dir == 0 ? "batch forward launch" : "batch backward launch");
@@ -1065,6 +1080,7 @@ sgdct_batch* sgdct_batch_create(sgdct_projector* const* slices, int nslice) {
       delete b;
       return nullptr;
     }
+    b->proj.push_back(slices[i]);
     SliceDesc d;
     d.g = h->args;
     d.basis = h->basis;
@@ -1117,7 +1133,7 @@ int sgdct_batch_layout(const sgdct_batch* b, long long* img_off, long long* sino
 }
 
 int sgdct_batch_config(sgdct_batch* b, int fwd_blocks_per_cu, int bwd_blocks_per_cu, int order) {
-  if (!b || fwd_blocks_per_cu < 1 || fwd_blocks_per_cu > 32 || bwd_blocks_per_cu < 1 || bwd_blocks_per_cu > 32 ||
+  if (!b || fwd_blocks_per_cu < 0 || fwd_blocks_per_cu > 32 || bwd_blocks_per_cu < 0 || bwd_blocks_per_cu > 32 ||
       (order != 0 && order != 1))
     return fail(SGDCT_EINVAL, "bad batch configuration");
   b->drop_graphs();

@@ -66,7 +66,7 @@ def batch(reps=3, tune_cache=None):
     rec["per_slice_ms"] = dict(A=_time(lambda: B.forward_per_slice(img, out_s), reps),
                                AT=_time(lambda: B.backward_per_slice(sino, out_i), reps))
     ref_s, ref_i = out_s.clone(), out_i.clone()
-    B.set_config(8, 4, 0)
+    B.set_config(8, 8, 0)
     rec["persistent_default_ms"] = dict(A=_time(lambda: B.forward(img, out_s), reps),
                                         AT=_time(lambda: B.backward(sino, out_i), reps))
     rec["persistent_vs_per_slice_max_rel"] = _maxrel(out_s, ref_s, out_i, ref_i)

@@ -54,6 +54,15 @@ def test_batched_equals_per_slice(cuda):
         torch.cuda.synchronize()
         assert _same(s, ref_s), (fb, bb, order)
         assert _same(i, ref_i), (fb, bb, order)
+    # blocks per CU 0: one launch per slice inside the batch call, bit for bit
+    # the per-projector launches; mixed with a persistent direction
+    for fb, bb in [(0, 0), (0, 8), (8, 0)]:
+        B.set_config(fb, bb, 0)
+        s = B.forward(img)
+        i = B.backward(sino)
+        torch.cuda.synchronize()
+        assert torch.equal(s, ref_s) if fb == 0 else _same(s, ref_s), (fb, bb)
+        assert torch.equal(i, ref_i) if bb == 0 else _same(i, ref_i), (fb, bb)
     kinds = {p.basis for p in B.projectors}
     assert kinds == {"B0", "B1", "SF"}
 
@@ -65,6 +74,7 @@ def test_native_graph_replay(cuda):
     img, sino = _inputs(B, cuda, seed=1)
     out_s, out_i = B.empty_sinos(), B.empty_images()
     stream = torch.cuda.Stream()
+    B.set_config(0, 8, 0)  # A captured as one launch per slice, A^T persistent
     with torch.cuda.stream(stream):
         for rep in range(3):
             B.forward(img, out=out_s, graph=True)
