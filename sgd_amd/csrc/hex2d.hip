@@ -368,6 +368,8 @@ __global__ __launch_bounds__(256) void hex_fwd_kernel(HexArgs ha, const float* _
   else sino[blockIdx.y * nsino + out] = (float)(acc * g.dx * g.dy);
 }
 
+constexpr int kHexH0Window = 72;
+
 // A^T: lattice-point-driven gather, 16 x 16 point tiles, LDS bin windows.
 // grid (tiles, tiles, batch * ngroup), H0's blocks mapped by xcd_block: z =
 // b * ngroup + k takes views
@@ -381,8 +383,12 @@ __global__ __launch_bounds__(256) void hex_bwd_kernel(HexArgs ha, const float* _
   using Bin = typename T::Bin;
   constexpr int BIN4 = sizeof(Bin) / 16;
   static_assert(sizeof(Bin) % 16 == 0, "bin records are whole float4s");
-  __shared__ float4 lds[kViewChunk][kMaxWindow][BIN4];
-  __shared__ float ldsy[kViewChunk][kMaxWindow];
+  // LDS bin window per view: H0's 80-byte records at 128 bins would hold 42
+  // KB per block (3 waves per SIMD); a 16 x 16 H0 tile spans up to ~66 bins
+  // at 0.3 mm per bin, and bins past the window are read from global memory
+  constexpr int kWin = HB == BASIS_H0 ? kHexH0Window : kMaxWindow;
+  __shared__ float4 lds[kViewChunk][kWin][BIN4];
+  __shared__ float ldsy[kViewChunk][kWin];
   __shared__ int win_lo[kViewChunk], win_n[kViewChunk];
   const Proj2DArgs& g = ha.g;
   // XCD-contiguous blocks (xcd_block) for H0: A^T 0.959 -> 0.939 ms (296 x
@@ -417,11 +423,11 @@ __global__ __launch_bounds__(256) void hex_bwd_kernel(HexArgs ha, const float* _
       }
       int lo = (int)fmax(floor(umin) - 1.0, 0.0), hi = (int)fmin(floor(umax) + 1.0, (double)(g.nu - 1));
       win_lo[threadIdx.x] = lo;
-      win_n[threadIdx.x] = hi >= lo ? min(hi - lo + 1, kMaxWindow) : 0;
+      win_n[threadIdx.x] = hi >= lo ? min(hi - lo + 1, kWin) : 0;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < nv * kMaxWindow; idx += blockDim.x) {
-      const int j = idx / kMaxWindow, i = idx - j * kMaxWindow;
+    for (int idx = threadIdx.x; idx < nv * kWin; idx += blockDim.x) {
+      const int j = idx / kWin, i = idx - j * kWin;
       if (i < win_n[j]) {
         const size_t gi = (size_t)(v0 + j) * g.nu + win_lo[j] + i;
         const float4* src = reinterpret_cast<const float4*>(bins + gi);
