@@ -85,6 +85,14 @@ static void view_placement(double angle, double lsd, double lso, double du, doub
  * reference builds the same map by matrix products and a 3x3 inverse, with a
  * special identity fallback on the 3pi/4, 7pi/4 diagonals that evaluates to
  * the same u; the closed form needs no fallback. */
+/* Index-space map terms as the reference forms them (utils/geo.cu:141-159):
+ * the grid step d and the centre offset (n - 1) d / 2 are float there, the
+ * offset a float product.  Reference mode keeps both roundings; exact mode
+ * evaluates them in FP64 like every other term (with d = 0.15 mm and
+ * n = 2048 the float offset alone moves the image by 6e-6 mm). */
+static double grid_step(double d) { return (double)(fpt)d; }
+static double centre_offset(int n, double d) { return (double)((fpt)(n - 1) * (fpt)d / (fpt)2.0f); }
+
 static void view_pm(const double *src, const double *dtv, const double *puv, double du, int nu,
                     int nx, int ny, double dx, double dy, double *pm, fpt *pmi) {
     double ex = dtv[0] - src[0], ey = dtv[1] - src[1];
@@ -103,14 +111,14 @@ static void view_pm(const double *src, const double *dtv, const double *puv, dou
         pm[4 + j] = row1[j] / d0;
         pm[8 + j] = row2[j] / d0;
     }
-    /* index-space version (utils/geo.cu:151-160; ox, oy in float there) */
-    fpt fox = (fpt)((nx - 1) * (float)dx / 2.0f);
-    fpt foy = (fpt)((ny - 1) * (float)dy / 2.0f);
+    /* index-space version (utils/geo.cu:151-160) */
+    const double sx = grid_step(dx), sy = grid_step(dy);
+    const double fox = centre_offset(nx, dx), foy = centre_offset(ny, dy);
     for (int r = 0; r < 3; ++r) {
-        pmi[4 * r + 0] = (fpt)(dx * pm[4 * r + 0]);
-        pmi[4 * r + 1] = (fpt)(dy * pm[4 * r + 1]);
+        pmi[4 * r + 0] = (fpt)(sx * pm[4 * r + 0]);
+        pmi[4 * r + 1] = (fpt)(sy * pm[4 * r + 1]);
         pmi[4 * r + 2] = (fpt)(1.0 * pm[4 * r + 2]);
-        pmi[4 * r + 3] = (fpt)(pm[4 * r + 3] - (double)fox * pm[4 * r + 0] - (double)foy * pm[4 * r + 1]);
+        pmi[4 * r + 3] = (fpt)(pm[4 * r + 3] - fox * pm[4 * r + 0] - foy * pm[4 * r + 1]);
     }
 }
 
@@ -1084,9 +1092,7 @@ void *or_geo3_create(int nx, int ny, int nz, int np, int nu, int nv, double dx, 
     t->g = (ogeo *)or_geo_create(nx, ny, np, nu, dx, dy, du, lsd, lso, angles, 0);
     t->nz = nz; t->nv = nv; t->dz = dz; t->dv = dv;
     t->pmv = (fpt *)malloc(sizeof(fpt) * 4 * np);
-    fpt fox = (fpt)((nx - 1) * (float)dx / 2.0f);
-    fpt foy = (fpt)((ny - 1) * (float)dy / 2.0f);
-    fpt foz = (fpt)((nz - 1) * (float)dz / 2.0f);
+    const double fox = centre_offset(nx, dx), foy = centre_offset(ny, dy), foz = centre_offset(nz, dz);
     for (int p = 0; p < np; ++p) {
         const double *src = t->g->src + 2 * p, *dtv = t->g->dtv + 2 * p;
         double ex = dtv[0] - src[0], ey = dtv[1] - src[1];
@@ -1098,10 +1104,10 @@ void *or_geo3_create(int nx, int ny, int nz, int np, int nu, int nv, double dx, 
         /* v = vc + (L/dv) z / ((X - src).e), rows scaled so pm[11] == 1 */
         double r1[4] = {vc * ex / d0, vc * ey / d0, (L / dv) / d0, vc};
         fpt *q = t->pmv + 4 * p;
-        q[0] = (fpt)(dx * r1[0]);
-        q[1] = (fpt)(dy * r1[1]);
-        q[2] = (fpt)(dz * r1[2]);
-        q[3] = (fpt)(r1[3] - (double)fox * r1[0] - (double)foy * r1[1] - (double)foz * r1[2]);
+        q[0] = (fpt)(grid_step(dx) * r1[0]);
+        q[1] = (fpt)(grid_step(dy) * r1[1]);
+        q[2] = (fpt)(grid_step(dz) * r1[2]);
+        q[3] = (fpt)(r1[3] - fox * r1[0] - foy * r1[1] - foz * r1[2]);
     }
     return t;
 }

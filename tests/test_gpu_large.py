@@ -3,10 +3,10 @@
 4x more views and bins than the headline workload, a 283 MB per-(view, bin)
 table.  For B1, B0 and SF:
   * A on four views spread over the circle and A^T of a sinogram holding
-    only those views, against the FP64 oracle: <= 1e-5 relative L2, or no
-    farther from it than the reference's own FP32 formulas (the oracle's FP32
-    build) are -- at this sampling SF's FP32 corner projections lose more
-    (FP32 build 8.2e-5 from FP64 in A^T, the kernel 2.5e-5);
+    only those views, against the FP64 oracle: <= 1e-5 relative L2 (the
+    distance of the reference's own FP32 formulas, the oracle's FP32 build,
+    is printed beside it: at this sampling their FP32 projection matrix alone
+    moves SF's trapezoids by ~1e-4 bins, 4e-5 .. 8e-5 in A^T);
   * the adjoint <A x, y> = <x, A^T y> (FP64 sums), <= 1e-6;
   * the automatic split launches against unsplit ones, <= 1e-6."""
 import numpy as np
@@ -46,14 +46,14 @@ def test_large_fan_beam(cuda, oracle_exact, oracle_ref, basis):
         y32 = gr.forward(basis, x, views=(p, p + 1))[p]
         e, e32 = rel(y[p], yo), rel(y32, yo)
         print("  A view %d rel %.2e (FP32 formulas %.2e)" % (p, e, e32), flush=True)
-        assert e <= max(1e-5, e32), (basis, p, e, e32)
+        assert e <= 1e-5, (basis, p, e, e32)
         s[p] = rng.random(NU)
         z_ref += g.backward(basis, s, views=(p, p + 1))
         z32 += gr.backward(basis, s, views=(p, p + 1))
     z = P.backward(torch.tensor(s, dtype=torch.float32, device=cuda)).double().cpu().numpy()
     e, e32 = rel(z, z_ref), rel(z32, z_ref)
     print("  A^T on %d views rel %.2e (FP32 formulas %.2e)" % (len(VIEWS), e, e32), flush=True)
-    assert e <= max(1e-5, e32)
+    assert e <= 1e-5, (basis, e, e32)
     # adjoint and split on full-size random data
     yr = torch.rand((NV, NU), device=cuda, dtype=torch.float32)
     ax = P.forward(xt)
