@@ -1,4 +1,4 @@
-"""The fan-beam projectors at a large size: 2048 x 2048 pixels of 0.15 mm,
+"""The 2-D projectors at a large size.  Fan-beam: 2048 x 2048 pixels of 0.15 mm,
 2880 views, 3072 bins of 0.2 mm (lso 500, lsd 1000 mm) -- a 16x larger image,
 4x more views and bins than the headline workload, a 283 MB per-(view, bin)
 table.  For B1, B0 and SF:
@@ -8,7 +8,9 @@ table.  For B1, B0 and SF:
     is printed beside it: at this sampling their FP32 projection matrix alone
     moves SF's trapezoids by ~1e-4 bins, 4e-5 .. 8e-5 in A^T);
   * the adjoint <A x, y> = <x, A^T y> (FP64 sums), <= 1e-6;
-  * the automatic split launches against unsplit ones, <= 1e-6."""
+  * the automatic split launches against unsplit ones, <= 1e-6.
+Hexagonal (H0, Hl): a 1184 x 1024 point lattice of 0.15 mm (Hl 2367 rows),
+2048 views, 2560 bins, the same checks (test_large_hex)."""
 import numpy as np
 import pytest
 
@@ -56,6 +58,65 @@ def test_large_fan_beam(cuda, oracle_exact, oracle_ref, basis):
     assert e <= 1e-5, (basis, e, e32)
     # adjoint and split on full-size random data
     yr = torch.rand((NV, NU), device=cuda, dtype=torch.float32)
+    ax = P.forward(xt)
+    aty = P.backward(yr)
+    lhs = float((ax.double() * yr.double()).sum())
+    rhs = float((xt.double() * aty.double()).sum())
+    e = abs(lhs - rhs) / abs(rhs)
+    print("  adjoint rel %.2e" % e, flush=True)
+    assert e <= 1e-6
+    P.set_split(1, 1)
+    ax1, aty1 = P.forward(xt), P.backward(yr)
+    torch.cuda.synchronize()
+    ea = float((ax1 - ax).double().norm() / ax.double().norm())
+    eb = float((aty1 - aty).double().norm() / aty.double().norm())
+    print("  split vs unsplit rel %.2e / %.2e" % (ea, eb), flush=True)
+    assert ea <= 1e-6 and eb <= 1e-6
+
+
+# Hexagonal lattices at a large size: 1184 rows x 1024 points of lattice
+# constant 0.15 mm (16x the points of the 296 x 256 workload; Hl's rows are
+# dt/2 apart, 2367 of them), 2048 views, 2560 bins of 0.2 mm.
+HNL, HNC, HNV, HNU, HDT = 1184, 1024, 2048, 2560, 0.15
+HVIEWS = (0, 171, 512, 1001, 1365)
+
+
+@pytest.mark.parametrize("basis", ["H0", "Hl"])
+def test_large_hex(cuda, oracle_exact, oracle_ref, basis):
+    """Sampled views of A and A^T (white-noise sinogram rows) against the
+    FP64 oracle, <= 1e-5; the adjoint, <= 1e-6; split against unsplit."""
+    import torch
+    from pyoracle import HexGeometry
+    from sgd_amd.torch_ops import Projector
+    nl = HNL if basis == "H0" else 2 * HNL - 1
+    ang = 2 * np.pi * np.arange(HNV) / HNV
+    print("large %s: %d x %d points, %d views, %d bins" % (basis, nl, HNC, HNV, HNU), flush=True)
+    P = Projector.hexagonal(basis, nl, HNC, ang, HNU, HDT, DU, LSD, LSO)
+    rng = np.random.default_rng(12)
+    x = rng.random((nl, HNC))
+    x[1::2, -1] = 0.0
+    xt = torch.tensor(x, dtype=torch.float32, device=cuda)
+    y = P.forward(xt).double().cpu().numpy()
+    g = HexGeometry(oracle_exact, nl, HNC, ang, HNU, HDT, DU, LSD, LSO)
+    gr = HexGeometry(oracle_ref, nl, HNC, ang, HNU, HDT, DU, LSD, LSO)
+    s = np.zeros((HNV, HNU))
+    z_ref = np.zeros((nl, HNC))
+    z32 = np.zeros((nl, HNC))
+    for p in HVIEWS:
+        yo = g.forward(basis, x, views=(p, p + 1))[p]
+        y32 = gr.forward(basis, x, views=(p, p + 1))[p]
+        e, e32 = rel(y[p], yo), rel(y32, yo)
+        print("  A view %d rel %.2e (FP32 formulas %.2e)" % (p, e, e32), flush=True)
+        assert e <= 1e-5, (basis, p, e, e32)
+        s[p] = rng.random(HNU)
+        z_ref += g.backward(basis, s, views=(p, p + 1))
+        z32 += gr.backward(basis, s, views=(p, p + 1))
+    z = P.backward(torch.tensor(s, dtype=torch.float32, device=cuda)).double().cpu().numpy()
+    e, e32 = rel(z, z_ref), rel(z32, z_ref)
+    print("  A^T on %d views rel %.2e (FP32 formulas %.2e)" % (len(HVIEWS), e, e32), flush=True)
+    assert e <= 1e-5, (basis, e, e32)
+    assert np.all(z[1::2, -1] == 0.0)
+    yr = torch.rand((HNV, HNU), device=cuda, dtype=torch.float32)
     ax = P.forward(xt)
     aty = P.backward(yr)
     lhs = float((ax.double() * yr.double()).sum())
