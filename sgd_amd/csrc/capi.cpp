@@ -190,6 +190,21 @@ struct sgdct_projector {
   hipStream_t cone_s2 = nullptr;
   hipEvent_t cone_ev[2] = {nullptr, nullptr};
 
+  // Frees the cone-beam workspace and clears its pointers: the workspace is
+  // either complete or absent, also after an allocation that failed halfway.
+  void release_cone() {
+    for (float** b : {&d_xT, &d_Y, &d_projT, &d_Y2}) {
+      if (*b) (void)hipFree(*b);
+      *b = nullptr;
+    }
+    if (cone_s2) (void)hipStreamDestroy(cone_s2);
+    cone_s2 = nullptr;
+    for (hipEvent_t& ev : cone_ev) {
+      if (ev) (void)hipEventDestroy(ev);
+      ev = nullptr;
+    }
+  }
+
   ~sgdct_projector() {
     if (d_views) (void)hipFree(d_views);
     if (d_bins) (void)hipFree(d_bins);
@@ -197,13 +212,7 @@ struct sgdct_projector {
     if (d_sino) (void)hipFree(d_sino);
     if (d_count) (void)hipFree(d_count);
     if (d_part) (void)hipFree(d_part);
-    if (d_xT) (void)hipFree(d_xT);
-    if (d_Y) (void)hipFree(d_Y);
-    if (d_projT) (void)hipFree(d_projT);
-    if (d_Y2) (void)hipFree(d_Y2);
-    if (cone_s2) (void)hipStreamDestroy(cone_s2);
-    for (hipEvent_t ev : cone_ev)
-      if (ev) (void)hipEventDestroy(ev);
+    release_cone();
     if (h_img) (void)hipHostFree(h_img);
     if (h_sino) (void)hipHostFree(h_sino);
   }
@@ -412,7 +421,18 @@ int ensure_staging(sgdct_projector* h) {
   if (e == hipSuccess) e = hipMalloc(&h->d_sino, ns * sizeof(float));
   if (e == hipSuccess) e = hipHostMalloc(&h->h_img, ni * sizeof(float));
   if (e == hipSuccess) e = hipHostMalloc(&h->h_sino, ns * sizeof(float));
-  if (e != hipSuccess) return hip_fail(e, "staging allocation");
+  if (e != hipSuccess) {
+    // all four or none: the guard above tests d_img only
+    for (float** b : {&h->d_img, &h->d_sino}) {
+      if (*b) (void)hipFree(*b);
+      *b = nullptr;
+    }
+    for (float** b : {&h->h_img, &h->h_sino}) {
+      if (*b) (void)hipHostFree(*b);
+      *b = nullptr;
+    }
+    return hip_fail(e, "staging allocation");
+  }
   return SGDCT_OK;
 }
 
@@ -452,7 +472,10 @@ int ensure_cone_workspace(sgdct_projector* h) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->cone_s2, hipStreamNonBlocking);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&h->cone_ev[k], hipEventDisableTiming);
   }
-  if (e != hipSuccess) return hip_fail(e, "cone-beam workspace");
+  if (e != hipSuccess) {
+    h->release_cone();
+    return hip_fail(e, "cone-beam workspace");
+  }
   return SGDCT_OK;
 }
 
@@ -476,8 +499,13 @@ int cone_forward(sgdct_projector* h, const float* vol, float* proj, int batch, h
       e = launch_cone_forward_chunk(h->cone_model, a, p0, std::min(h->chunk, h->np - p0), h->d_xT,
                                     odd ? h->d_Y2 : h->d_Y, y, odd ? h->cone_s2 : s);
     }
-    if (two && e == hipSuccess) e = hipEventRecord(h->cone_ev[1], h->cone_s2);
-    if (two && e == hipSuccess) e = hipStreamWaitEvent(s, h->cone_ev[1], 0);
+    if (two) {
+      // joined also after a failed launch: no pass of this call stays
+      // unordered against the caller's stream
+      hipError_t j = hipEventRecord(h->cone_ev[1], h->cone_s2);
+      if (j == hipSuccess) j = hipStreamWaitEvent(s, h->cone_ev[1], 0);
+      if (e == hipSuccess) e = j;
+    }
     if (e != hipSuccess) return hip_fail(e, "cone-beam forward launch");
   }
   return SGDCT_OK;
