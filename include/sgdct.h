@@ -242,8 +242,10 @@ int sgdct_tv_descent(float *x, float *g, double *work, const double *step, int n
  * Many B0 / B1 / SF projectors of uneven size, view count and basis run as
  * one persistent launch per direction over a device work queue (A: 64-bin
  * waves of one view, A^T: 16x16 pixel tiles), slices ordered by decreasing
- * per-item cost.  Results are bit-identical to the per-projector entry
- * points.  Images / sinograms of all slices are packed back to back in one
+ * per-item cost.  Results equal the per-projector entry points to one FP32
+ * rounding per element (those may split a walk or the views into FP64
+ * partial sums, associated differently), and bit for bit in the per-slice
+ * mode of sgdct_batch_config.  Images / sinograms of all slices are packed back to back in one
  * device buffer each, at the element offsets sgdct_batch_layout returns
  * (nslice + 1 entries each).  The projectors must outlive the batch; calls on
  * one batch must not overlap (they share the queue counters). */
