@@ -11,7 +11,8 @@
 // Items are ordered slice by slice, slices by decreasing per-item cost (the
 // longest-processing-time-first rule), so the big slices start first and
 // the small ones fill the tail.  Every item runs the same device function as
-// the per-slice kernels, so results are bit-identical to them.  Counters are
+// the per-slice kernels, so results equal their unsplit launches bit for bit
+// (a split launch's FP64 partial sums can differ by one FP32 rounding).  Counters are
 // cleared by a one-block kernel launched just before (no memset node, so the
 // pair can be captured in a graph and replayed).  The queue ends when the
 // counter passes the item count, which every wave / block reaches.
