@@ -46,7 +46,10 @@ __global__ __launch_bounds__(256) void batch_fwd_kernel(BatchPlan P, const float
   for (;;) {
     unsigned int mine = 0;
     if (lane == 0) mine = atomicAdd(&P.counters[0], 1u);
-    const long long it = (long long)(unsigned int)__shfl((int)mine, 0);
+    // lane 0's value, wave-uniform for the compiler: the slice lookup and
+    // descriptor then use scalar loads and registers (through a lane shuffle
+    // the kernel held 100 VGPRs, with this 92)
+    const long long it = (long long)(unsigned int)__builtin_amdgcn_readfirstlane((int)mine);
     if (it >= P.nfwd) break;
     const int k = find_slice(P.fwd_prefix, P.nslice, it);
     const SliceDesc d = P.slices[P.fwd_order[k]];
@@ -68,7 +71,9 @@ __global__ __launch_bounds__(256, 6) void batch_bwd_kernel(BatchPlan P, const fl
   for (;;) {
     if (threadIdx.x == 0) s_it = atomicAdd(&P.counters[1], 1u);
     __syncthreads();
-    const long long it = (long long)s_it;
+    // the same value in every lane; wave-uniform for the compiler (scalar
+    // slice lookup and descriptor)
+    const long long it = (long long)(unsigned int)__builtin_amdgcn_readfirstlane((int)s_it);
     __syncthreads();
     if (it >= P.nbwd) break;
     const int k = find_slice(P.bwd_prefix, P.nslice, it);
