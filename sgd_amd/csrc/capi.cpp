@@ -443,7 +443,7 @@ Cone3DArgs cone_args(const sgdct_projector* h) {
   a.nv = h->nv;
   a.nrc = (h->nv + 63) / 64;
   a.vc = (float)(0.5 * h->nv);
-  a.ldz = (float)(h->lsd * h->dz / h->dv);
+  a.ldz = h->lsd * h->dz / h->dv;
   a.cvol = (float)(h->dx * h->dy * h->dz);
   a.dz = (float)h->dz;
   a.du = h->du;

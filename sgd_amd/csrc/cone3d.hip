@@ -88,8 +88,9 @@ __device__ __forceinline__ double recip(double d) {
 }
 
 struct ColumnAxial {
-  float ka, kb;   // ldz / depth at the two x corners (SF-TR: both the centre)
-  float r2;       // |in-plane source -> centre|^2
+  float ka, kb;        // ldz / depth at the two x corners (SF-TR: both the centre)
+  float ka_lo, kb_lo;  // their FP32 remainders: ka + ka_lo is the FP64 value
+  float r2;            // |in-plane source -> centre|^2
 };
 
 template <int M>
@@ -101,18 +102,22 @@ __device__ __forceinline__ ColumnAxial column_axial(const Cone3DArgs& a, const V
   // depth along the central axis = -cross(pu, X - p) / |pu|, so
   // ldz / depth = ldz |pu| / cross(p - X, pu) with |pu| = du
   const double num = a.ldz * a.du;
+  const double ka = num * recip(vg.puy * wxa - vg.pux * wy);
+  const double kb = num * recip(vg.puy * wxb - vg.pux * wy);
   ColumnAxial c;
-  c.ka = (float)(num * recip(vg.puy * wxa - vg.pux * wy));
-  c.kb = (float)(num * recip(vg.puy * wxb - vg.pux * wy));
+  c.ka = (float)ka;
+  c.kb = (float)kb;
+  c.ka_lo = (float)(ka - (double)c.ka);
+  c.kb_lo = (float)(kb - (double)c.kb);
   c.r2 = (float)(wx * wx + wy * wy);
   return c;
 }
 
+// Trapezoid of voxel zc (index offset from the column's middle) from the rows
+// of its corners: a1, a2 at x corner a and z -+ 1/2, b1, b2 at corner b.
 template <int M>
-__device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const ColumnAxial& c, int iz) {
-  const float zc = (float)iz - 0.5f * (float)(a.nz - 1);
-  const float a1 = __builtin_fmaf(c.ka, zc - 0.5f, a.vc), a2 = __builtin_fmaf(c.ka, zc + 0.5f, a.vc);
-  const float b1 = __builtin_fmaf(c.kb, zc - 0.5f, a.vc), b2 = __builtin_fmaf(c.kb, zc + 0.5f, a.vc);
+__device__ __forceinline__ AxialTrap trap_from_corners(const Cone3DArgs& a, const ColumnAxial& c, float zc, float a1,
+                                                       float a2, float b1, float b2) {
   AxialTrap t;
   t.v0 = fminf(a1, b1);
   t.v3 = fmaxf(a2, b2);
@@ -134,6 +139,35 @@ __device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const Colum
   t.i01 = inv_or_zero(t.v1 - t.v0);
   t.i23 = inv_or_zero(t.v3 - t.v2);
   return t;
+}
+
+// Corners as detector rows (A's staged records).
+template <int M>
+__device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const ColumnAxial& c, int iz) {
+  const float zc = (float)iz - 0.5f * (float)(a.nz - 1);
+  const float a1 = __builtin_fmaf(c.ka, zc - 0.5f, a.vc), a2 = __builtin_fmaf(c.ka, zc + 0.5f, a.vc);
+  const float b1 = __builtin_fmaf(c.kb, zc - 0.5f, a.vc), b2 = __builtin_fmaf(c.kb, zc + 0.5f, a.vc);
+  return trap_from_corners<M>(a, c, zc, a1, a2, b1, b2);
+}
+
+// The same trapezoid with its corners relative to the integer row *k0 next
+// to it.  As FP32 detector rows in the hundreds, corners carry up to ~6e-5
+// rows of rounding, which A^T of white-noise rows reads (~8e-6 relative at
+// 512^3 voxels / 768 rows); relative to k0, each corner is one fused
+// multiply-add of the exact vc - k0 plus the slope's FP32 remainder, ~1e-7
+// rows from exact.
+template <int M>
+__device__ __forceinline__ AxialTrap voxel_trap_rel(const Cone3DArgs& a, const ColumnAxial& c, int iz, int* k0) {
+  const float zc = (float)iz - 0.5f * (float)(a.nz - 1);
+  const float zl = zc - 0.5f, zh = zc + 0.5f;
+  const int k = (int)floorf(__builtin_fmaf(c.ka, zl, a.vc));
+  const float vk = a.vc - (float)k;  // exact: both are multiples of 1/2 below 2^23
+  const float a1 = __builtin_fmaf(c.ka_lo, zl, __builtin_fmaf(c.ka, zl, vk));
+  const float a2 = __builtin_fmaf(c.ka_lo, zh, __builtin_fmaf(c.ka, zh, vk));
+  const float b1 = __builtin_fmaf(c.kb_lo, zl, __builtin_fmaf(c.kb, zl, vk));
+  const float b2 = __builtin_fmaf(c.kb_lo, zh, __builtin_fmaf(c.kb, zh, vk));
+  *k0 = k;
+  return trap_from_corners<M>(a, c, zc, a1, a2, b1, b2);
 }
 
 // detector row j spans [j + row_off, j + row_off + 1)
@@ -533,17 +567,20 @@ __global__ __launch_bounds__(256) void axial_bwd_kernel(Cone3DArgs a, int p0, in
     for (int s = 0; s < kSlots; ++s) {
       const int iz = z0 + s * 64 + lane;
       if (iz >= a.nz) continue;
-      const AxialTrap t = voxel_trap<M>(a, ca, iz);
+      int k0;
+      const AxialTrap t = voxel_trap_rel<M>(a, ca, iz, &k0);
       const float ro = row_off<M>();
-      int jlo = (int)fminf(fmaxf(floorf(t.v0 - ro), 0.0f), lastj);
-      int jhi = (int)fminf(fmaxf(floorf(t.v3 - ro), 0.0f), lastj);
+      // rows relative to k0 (small integers plus ro: exact)
+      int jlo = (int)fminf(fmaxf(floorf(t.v0 - ro) + (float)k0, 0.0f), lastj);
+      int jhi = (int)fminf(fmaxf(floorf(t.v3 - ro) + (float)k0, 0.0f), lastj);
       if (BIG) {
         jlo = max(jlo, r0);
         jhi = min(jhi, r0 + nr - 1);
       }
       float sum = 0.0f;
-      for (int j = jlo; j <= jhi; ++j) {
-        const float gm = trap_gamma((float)j + ro, t);
+      float sj = (float)(jlo - k0) + ro;
+      for (int j = jlo; j <= jhi; ++j, sj += 1.0f) {
+        const float gm = trap_gamma(sj, t);
         if (keep_gamma<M>(gm)) sum = __builtin_fmaf(gm, yl[j - r0], sum);
       }
       acc[s] += (double)(t.cf * sum);

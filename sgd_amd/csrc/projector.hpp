@@ -51,7 +51,7 @@ struct Cone3DArgs {
   Proj2DArgs g;     // in-plane geometry and B1 bin records (h = 1)
   int nz, nv, nrc;  // nrc = ceil(nv / 64)
   float vc;         // detector row of the source plane (nv / 2)
-  float ldz;        // lsd * dz / dv: v = vc + ldz * (z index offset) / depth
+  double ldz;       // lsd * dz / dv: v = vc + ldz * (z index offset) / depth
   float cvol;       // dx * dy * dz
   float dz;
   double du;        // |pu|

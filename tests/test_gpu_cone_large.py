@@ -6,9 +6,10 @@ the volume's cone near the source, so rows are also cut at its edges.  For
 B1 (A_B1_3D.cu) and SF-TR (SF_projector.cu, fixed):
   * A on two views and A^T of a projection set holding only those views,
     against the FP64 oracle: <= 1e-5 relative L2 (A^T of white-noise rows is
-    the sensitive one: the axial trapezoids' corners are FP32 row
-    coordinates up to ~800, ~3e-5 rows apart from exact; the reference's own
-    FP32 formulas, the oracle's FP32 build, are printed beside it);
+    the sensitive one: as FP32 detector rows up to ~800 the axial
+    trapezoids' corners would be ~3e-5 rows from exact, which A^T's
+    per-voxel anchoring avoids; the reference's own FP32 formulas, the
+    oracle's FP32 build, are printed beside it, 1.1e-5 .. 1.7e-5);
   * the adjoint <A x, y> = <x, A^T y> (FP64 sums) on full-size random data,
     <= 1e-6."""
 import numpy as np
