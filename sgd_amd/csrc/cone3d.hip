@@ -159,13 +159,13 @@ __device__ __forceinline__ AxialTrap voxel_trap(const Cone3DArgs& a, const Colum
 template <int M>
 __device__ __forceinline__ AxialTrap voxel_trap_rel(const Cone3DArgs& a, const ColumnAxial& c, int iz, int* k0) {
   const float zc = (float)iz - 0.5f * (float)(a.nz - 1);
-  const float zl = zc - 0.5f, zh = zc + 0.5f;
+  const float zl = zc - 0.5f;
   const int k = (int)floorf(__builtin_fmaf(c.ka, zl, a.vc));
   const float vk = a.vc - (float)k;  // exact: both are multiples of 1/2 below 2^23
   const float a1 = __builtin_fmaf(c.ka_lo, zl, __builtin_fmaf(c.ka, zl, vk));
-  const float a2 = __builtin_fmaf(c.ka_lo, zh, __builtin_fmaf(c.ka, zh, vk));
   const float b1 = __builtin_fmaf(c.kb_lo, zl, __builtin_fmaf(c.kb, zl, vk));
-  const float b2 = __builtin_fmaf(c.kb_lo, zh, __builtin_fmaf(c.kb, zh, vk));
+  // the far corners are one slope further (zc + 1/2 = zl + 1)
+  const float a2 = a1 + c.ka, b2 = b1 + c.kb;
   *k0 = k;
   return trap_from_corners<M>(a, c, zc, a1, a2, b1, b2);
 }
