@@ -3,7 +3,7 @@
 8x the voxels and 4x the detector cells of the 256^3 workload, 805 MB of row
 profiles per view, so a pass holds two views).  The detector is shorter than
 the volume's cone near the source, so rows are also cut at its edges.  For
-B1 (A_B1_3D.cu) and SF-TR (SF_projector.cu, fixed):
+B1 (A_B1_3D.cu), SF-TR and SF-TT (SF_projector.cu, fixed):
   * A on two views and A^T of a projection set holding only those views,
     against the FP64 oracle: <= 1e-5 relative L2 (A^T of white-noise rows is
     the sensitive one: as FP32 detector rows up to ~800 the axial
@@ -26,7 +26,7 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 
 
-@pytest.mark.parametrize("model", ["B1", "SF-TR"])
+@pytest.mark.parametrize("model", ["B1", "SF-TR", "SF-TT"])
 def test_large_cone(cuda, oracle_exact, oracle_ref, model):
     import torch
     from pyoracle import Geometry3
@@ -36,9 +36,10 @@ def test_large_cone(cuda, oracle_exact, oracle_ref, model):
     P = Projector.cone_beam(N, N, N, ang, NU, NVR, D, D, D, DU, DU, LSD, LSO, model=model)
     G = Geometry3(oracle_exact, N, N, N, ang, NU, NVR, D, D, D, DU, DU, LSD, LSO)
     G32 = Geometry3(oracle_ref, N, N, N, ang, NU, NVR, D, D, D, DU, DU, LSD, LSO)
-    fwd = G.forward if model == "B1" else (lambda v, views: G.sf_forward(v, trap_v=False, views=views))
-    bwd = G.backward if model == "B1" else (lambda p, views: G.sf_backward(p, trap_v=False, views=views))
-    bwd32 = G32.backward if model == "B1" else (lambda p, views: G32.sf_backward(p, trap_v=False, views=views))
+    tv = model == "SF-TT"
+    fwd = G.forward if model == "B1" else (lambda v, views: G.sf_forward(v, trap_v=tv, views=views))
+    bwd = G.backward if model == "B1" else (lambda p, views: G.sf_backward(p, trap_v=tv, views=views))
+    bwd32 = G32.backward if model == "B1" else (lambda p, views: G32.sf_backward(p, trap_v=tv, views=views))
     rng = np.random.default_rng(21)
     x = rng.random((N, N, N), dtype=np.float32)
     xt = torch.from_numpy(x).to(cuda)
