@@ -99,7 +99,12 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// The failing call also left `e` as the runtime's last error, which the next
+// launch check (hipGetLastError after a kernel launch) would report as its
+// own; it is taken here so that a failed allocation does not fail the next
+// call as well.
 int hip_fail(hipError_t e, const char* what) {
+  (void)hipGetLastError();
   return fail(e == hipErrorOutOfMemory ? SGDCT_ENOMEM : SGDCT_EDEVICE,
               std::string(what) + ": " + hipGetErrorString(e));
 }
