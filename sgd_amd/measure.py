@@ -12,9 +12,11 @@
                                       # 64 -> 64 channels, batch 8 at 512 x 512)
     python -m sgd_amd.measure hexangle  # hexagonal A by view angle (1024 views within
                                         # 0.06 degrees of each angle, batch 8)
+    python -m sgd_amd.measure hostabi   # the B1 pair through the reference-named host
+                                        # entry points (FP64 host arrays, PCIe included)
 
 Times are HIP-event averages on the launch stream with inputs resident in
-device memory.
+device memory, except hostabi's (wall clock per call, host arrays in and out).
 """
 import argparse
 import json
@@ -115,6 +117,36 @@ def fan(reps=10):
                          A_evals_per_s=evals * batch / (a_ms * 1e-3)))
         del P
     return recs
+
+
+def hostabi(reps=20):
+    """The B1 pair at 512^2 (0.6 mm), 720 views, 768 bins (0.8 mm) through the
+    reference-named host entry points (b1_forward_projection /
+    b1_backward_projection via the pylops-compatible A_B1_operator): FP64 host
+    arrays in and out, so each call includes the host-side rounding to FP32,
+    the upload through pinned buffers, the kernels, the download and the
+    widening back to FP64.  Wall clock per call after 5 untimed calls."""
+    from .operators import A_B1_operator
+    nx = ny = 512
+    nv, nu = 720, 768
+    op = A_B1_operator(nx, ny, nv, nu, 0.6, 0.6, 0.8, 1000.0, 500.0, 2 * np.pi * np.arange(nv) / nv)
+    rng = np.random.default_rng(0)
+    x, y = rng.random(nx * ny), rng.random(nv * nu)
+
+    def wall(fn):
+        for _ in range(5):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    a_ms, at_ms = wall(lambda: op._matvec(x)), wall(lambda: op._rmatvec(y))
+    op.close()
+    return dict(workload="B1 fan beam, 512x512 (0.6 mm), 720 views, 768 bins (0.8 mm), FP64 host arrays through "
+                         "b1_forward_projection / b1_backward_projection",
+                A_ms=a_ms, AT_ms=at_ms, pairs_per_s=1e3 / (a_ms + at_ms),
+                host_bytes_per_pair=8 * 2 * (nx * ny + nv * nu), pcie_bytes_per_pair=4 * 2 * (nx * ny + nv * nu))
 
 
 def hexangle(reps=5, angles=(0, 5, 10, 15, 20, 25, 30, 45, 60, 90, 120, 150, 165)):
@@ -227,7 +259,7 @@ def unet(reps=5, batch_size=8, size=512):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("what", choices=["fan", "asd", "batch", "cone", "unet", "conv", "hexangle"])
+    ap.add_argument("what", choices=["fan", "asd", "batch", "cone", "unet", "conv", "hexangle", "hostabi"])
     ap.add_argument("--tune-cache", default=None)
     a = ap.parse_args(argv)
     if a.what == "batch":
@@ -240,6 +272,8 @@ def main(argv=None):
         recs = fan()
     elif a.what == "hexangle":
         recs = hexangle()
+    elif a.what == "hostabi":
+        recs = [hostabi()]
     elif a.what == "asd":
         recs = [asd()]
     else:
